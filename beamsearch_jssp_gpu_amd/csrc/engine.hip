@@ -1,0 +1,639 @@
+// engine.hip -- single-instance solves: workspace planning, the level loop,
+// the exact (host-replay) modes and the C ABI entry points around them.
+//
+// Reference counterparts: run_gpu_beam_search_v1 (gpu/v1/solver.cu:178-298),
+// run_gpu_beam_search_v2 (gpu/v2_twophases/solver.cu:256-386),
+// beam_search_ich (cpu/v0_ich/solver.cpp:8-54), beam_search_pms
+// (cpu/v1_pms/solver.cpp:8-70).  Differences by design: the device arena and
+// stream live in the engine and are reused across solves (the reference
+// mallocs and frees nine buffers per call), the level loop enqueues a fixed
+// launch sequence with every count kept on the device (the reference sizes
+// grids from min(P*n, K) on the host), and errors are returned, never exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "engine.hpp"
+#include "level_kernels.hpp"
+#include "snapshot.hpp"
+
+namespace jssp {
+
+int set_error(jssp_engine* e, int code, const char* where, const std::string& msg) {
+  if (e) e->err = std::string(where) + ": " + msg;
+  return code;
+}
+
+int ensure_device_arena(jssp_engine* e, size_t bytes) {
+  if (bytes <= e->dcap) return JSSP_OK;
+  if (e->dbuf) {
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+    JSSP_HIP(e, hipFree(e->dbuf));
+    e->dbuf = nullptr;
+    e->dcap = 0;
+  }
+  size_t cap = bytes + bytes / 4;
+  if (hipMalloc(&e->dbuf, cap) != hipSuccess) {
+    e->dbuf = nullptr;
+    (void)hipGetLastError();
+    return set_error(e, JSSP_ERR_NOMEM, "hipMalloc", "device arena of " + std::to_string(cap) + " bytes");
+  }
+  e->dcap = cap;
+  return JSSP_OK;
+}
+
+int ensure_host_staging(jssp_engine* e, size_t bytes) {
+  if (bytes <= e->hcap) return JSSP_OK;
+  if (e->hbuf) {
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+    JSSP_HIP(e, hipHostFree(e->hbuf));
+    e->hbuf = nullptr;
+    e->hcap = 0;
+  }
+  size_t cap = bytes + bytes / 4 + 4096;
+  if (hipHostMalloc(&e->hbuf, cap, hipHostMallocDefault) != hipSuccess) {
+    e->hbuf = nullptr;
+    (void)hipGetLastError();
+    return set_error(e, JSSP_ERR_NOMEM, "hipHostMalloc", "pinned staging of " + std::to_string(cap) + " bytes");
+  }
+  e->hcap = cap;
+  return JSSP_OK;
+}
+
+int check_instance(jssp_engine* e, const jssp_instance* in) {
+  if (!in || !in->machine || !in->duration) return set_error(e, JSSP_ERR_INVALID, "instance", "null instance");
+  const int n = in->n_jobs, m = in->n_machines;
+  if (n <= 0 || m <= 0) return set_error(e, JSSP_ERR_INVALID, "instance", "n_jobs and n_machines must be >= 1");
+  if (n > JSSP_MAX_JOBS || m > JSSP_MAX_MACHINES)
+    return set_error(e, JSSP_ERR_LIMIT, "instance", "n_jobs <= 128 and n_machines <= 255 required");
+  long long total = 0;
+  for (int i = 0; i < n * m; ++i) {
+    if (in->machine[i] < 0 || in->machine[i] >= m)
+      return set_error(e, JSSP_ERR_INVALID, "instance", "machine id out of range at op " + std::to_string(i));
+    if (in->duration[i] < 0)
+      return set_error(e, JSSP_ERR_INVALID, "instance", "negative duration at op " + std::to_string(i));
+    total += in->duration[i];
+  }
+  if (total >= (1ll << 30)) return set_error(e, JSSP_ERR_LIMIT, "instance", "sum of durations must be < 2^30");
+  return JSSP_OK;
+}
+
+int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p) {
+  if (!p) return set_error(e, JSSP_ERR_INVALID, "params", "null params");
+  if (p->beam_width < 1) return set_error(e, JSSP_ERR_INVALID, "params", "beam_width must be >= 1");
+  if (p->beam_width > JSSP_MAX_BEAM) return set_error(e, JSSP_ERR_LIMIT, "params", "beam_width > 65536");
+  if (p->mode < JSSP_MODE_ICH_EXACT || p->mode > JSSP_MODE_PMS_EXACT)
+    return set_error(e, JSSP_ERR_INVALID, "params", "unknown mode");
+  if (p->phases != 0 && p->phases != 1 && p->phases != 2)
+    return set_error(e, JSSP_ERR_INVALID, "params", "phases must be 1 or 2");
+  if ((long long)p->beam_width * in->n_jobs >= (1ll << 31))
+    return set_error(e, JSSP_ERR_LIMIT, "params", "beam_width * n_jobs must be < 2^31");
+  return JSSP_OK;
+}
+
+namespace {
+
+// Device workspace of one single-instance solve.
+struct Work {
+  int2* ops;
+  int* beam[2];
+  int* cbuf;
+  unsigned long long* keys;
+  uint8_t* cjob;
+  unsigned long long* bucketed;
+  uint32_t* hist[2];
+  uint32_t* fill[2];
+  uint32_t* excl;
+  BlockStat* bstat;
+  SelState* sel;
+  int* pcount;
+  int* trail;
+  unsigned long long* expanded;
+  uint32_t* lvl_exp;
+  int* seq;
+  int* result;
+  int* ex_cmax;
+  uint8_t* ex_job;
+  uint8_t* ex_mc;
+  int* ex_rcount;
+  int* ex_order;
+};
+
+struct Shape {
+  int n, m, S, B, K, L;
+  bool single, exact;
+  int nblk_score, nblk_sel, nblk_rank;
+};
+
+Shape make_shape(const jssp_instance* in, const jssp_params* p) {
+  Shape s;
+  s.n = in->n_jobs; s.m = in->n_machines;
+  s.S = 2 * s.n + s.m + 1;
+  s.B = s.n;
+  s.K = p->beam_width;
+  s.L = s.n * s.m;
+  s.exact = (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT);
+  s.single = !s.exact && p->phases == 1;
+  const long long N = (long long)s.K * s.B;
+  s.nblk_score = (int)std::max(1, std::min((s.K + kScoreWaves - 1) / kScoreWaves, 512));
+  s.nblk_sel = (int)std::max(1ll, std::min((N + 4095) / 4096, 256ll));
+  s.nblk_rank = (int)std::max(1, std::min((s.K + kRankWaves - 1) / kRankWaves, 2048));
+  return s;
+}
+
+size_t carve(const Shape& s, char* base, Work& w) {
+  Bump b;
+  b.base = base;
+  const size_t N = (size_t)s.K * s.B;
+  w.ops = b.take<int2>((size_t)s.n * s.m);
+  w.beam[0] = b.take<int>((size_t)s.K * s.S);
+  w.beam[1] = b.take<int>((size_t)s.K * s.S);
+  w.cbuf = s.single ? b.take<int>(N * s.S) : nullptr;
+  w.keys = b.take<unsigned long long>(std::max(N, (size_t)2 * s.K));
+  w.cjob = b.take<uint8_t>(N);
+  w.bucketed = b.take<unsigned long long>(N);
+  w.hist[0] = b.take<uint32_t>(kNB);
+  w.hist[1] = b.take<uint32_t>(kNB);
+  w.fill[0] = b.take<uint32_t>(kNB);
+  w.fill[1] = b.take<uint32_t>(kNB);
+  w.excl = b.take<uint32_t>(kNB);
+  w.bstat = b.take<BlockStat>(s.nblk_score);
+  w.sel = b.take<SelState>(1);
+  w.pcount = b.take<int>(s.L + 1);
+  w.trail = b.take<int>((size_t)std::max(1, s.L - 1) * s.K);
+  w.expanded = b.take<unsigned long long>(1);
+  w.lvl_exp = b.take<uint32_t>(s.L);
+  w.seq = b.take<int>(s.L);
+  w.result = b.take<int>(4);
+  if (s.exact) {
+    w.ex_cmax = b.take<int>(N);
+    w.ex_job = b.take<uint8_t>(N);
+    w.ex_mc = b.take<uint8_t>(N);
+    w.ex_rcount = b.take<int>(s.K);
+    w.ex_order = b.take<int>(s.K);
+  } else {
+    w.ex_cmax = nullptr; w.ex_job = nullptr; w.ex_mc = nullptr; w.ex_rcount = nullptr; w.ex_order = nullptr;
+  }
+  return b.off + 256;
+}
+
+LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
+  LevelArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.ops = w.ops;
+  a.beam_cur = w.beam[0];
+  a.beam_next = w.beam[1];
+  a.cbuf = w.cbuf;
+  a.keys = w.keys;
+  a.cjob = w.cjob;
+  a.bucketed = w.bucketed;
+  a.hist = w.hist[0]; a.fill = w.fill[0];
+  a.hist_next = w.hist[1]; a.fill_next = w.fill[1];
+  a.excl = w.excl;
+  a.bstat = w.bstat;
+  a.sel = w.sel;
+  a.pcount = w.pcount;
+  a.trail = w.trail;
+  a.expanded = w.expanded;
+  a.lvl_exp = w.lvl_exp;
+  a.seq_out = w.seq;
+  a.result = w.result;
+  a.ex_cmax = w.ex_cmax; a.ex_job = w.ex_job; a.ex_mc = w.ex_mc;
+  a.ex_rcount = w.ex_rcount; a.ex_order = w.ex_order;
+  a.n = s.n; a.m = s.m; a.S = s.S; a.B = s.B; a.K = s.K; a.L = s.L;
+  a.level = 0;
+  a.best_known = p->best_known;
+  a.nblk_score = s.nblk_score;
+  a.lseed = p->seed;
+  a.bt_parent = -1; a.bt_job = -1;
+  return a;
+}
+
+void set_level(LevelArgs& a, const Work& w, int level, uint32_t seed) {
+  a.level = level;
+  a.lseed = seed + (uint32_t)level;
+  a.beam_cur = w.beam[level & 1];
+  a.beam_next = w.beam[(level + 1) & 1];
+  a.hist = w.hist[level & 1];
+  a.fill = w.fill[level & 1];
+  a.hist_next = w.hist[(level + 1) & 1];
+  a.fill_next = w.fill[(level + 1) & 1];
+}
+
+// Per-kernel HIP-event timing (optional).
+struct Prof {
+  jssp_engine* e;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  size_t next = 0;
+  hipEvent_t get() {
+    if (next >= e->pev.size()) {
+      hipEvent_t ev;
+      if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+      e->pev.push_back(ev);
+    }
+    return e->pev[next++];
+  }
+  hipEvent_t begin() {
+    if (!e->profiling) return nullptr;
+    hipEvent_t a = get();
+    if (a) (void)hipEventRecord(a, e->stream);
+    return a;
+  }
+  void end(int kind, hipEvent_t a) {
+    if (!e->profiling || !a) return;
+    hipEvent_t b = get();
+    if (!b) return;
+    (void)hipEventRecord(b, e->stream);
+    marks.push_back({kind, {a, b}});
+  }
+  void collect() {
+    for (auto& mk : marks) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, mk.second.first, mk.second.second) == hipSuccess) {
+        e->kt[mk.first].ms += ms;
+        e->kt[mk.first].launches += 1;
+      }
+    }
+  }
+};
+
+int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops) {
+  const int total = in->n_jobs * in->n_machines;
+  int rc = ensure_host_staging(e, (size_t)total * sizeof(int2));
+  if (rc) return rc;
+  int2* h = reinterpret_cast<int2*>(e->hbuf);
+  for (int i = 0; i < total; ++i) { h[i].x = in->machine[i]; h[i].y = in->duration[i]; }
+  JSSP_HIP(e, hipMemcpyAsync(d_ops, h, (size_t)total * sizeof(int2), hipMemcpyHostToDevice, e->stream));
+  // The staging buffer is reused below; finish the copy first.
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  return JSSP_OK;
+}
+
+void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int>& pc,
+                           const std::vector<uint32_t>& lexp) {
+  // Algorithmic bytes per launch: the parent snapshots read (4S each), the
+  // children written (4S each, single-phase) and the 8-byte (score, child)
+  // keys; selection reads every key once per pass and writes the survivors;
+  // materialisation reads a parent and writes a child snapshot, the trail
+  // entry and reads the survivor key.
+  const double S4 = 4.0 * s.S;
+  for (int l = 0; l < s.L; ++l) {
+    const double P = pc[l], C = lexp[l];
+    const double Kp = (double)pc[l + 1];
+    if (s.single) e->kt[kKindScore].bytes += S4 * P + (S4 + 8.0) * C;
+    else e->kt[kKindScore].bytes += S4 * P + 8.0 * C;
+    if (l == s.L - 1) break;
+    e->kt[kKindHist].bytes += 8.0 * C;
+    e->kt[kKindPartition].bytes += 8.0 * C + 8.0 * Kp;
+    // the threshold-bin sort moves a few KB; its bytes are not tracked
+    e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 12.0 * Kp;
+  }
+}
+
+// PMS_GPU / ICH_DET, one beam.
+int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, uint32_t seed,
+                 int bound, jssp_result* res, int32_t* trail_out, int32_t* pcount_out) {
+  jssp_params q = *p;
+  q.seed = seed;
+  q.best_known = bound;
+  const Shape s = make_shape(in, &q);
+  Work w;
+  const size_t need = carve(s, nullptr, w);
+  int rc = ensure_device_arena(e, need);
+  if (rc) return rc;
+  carve(s, e->dbuf, w);
+  rc = upload_ops(e, in, w.ops);
+  if (rc) return rc;
+  LevelArgs a = base_args(s, w, &q);
+  const int mode = (q.mode == JSSP_MODE_ICH_DET) ? 1 : 2;
+  const int order = (mode == 1) ? kOrderErase : kOrderSwap;
+  Prof prof{e, {}, 0};
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  JSSP_HIP(e, launch_init(a, e->stream));
+  for (int l = 0; l < s.L; ++l) {
+    set_level(a, w, l, seed);
+    hipEvent_t t0 = prof.begin();
+    JSSP_HIP(e, launch_score(a, mode, s.single, e->stream));
+    prof.end(kKindScore, t0);
+    if (l == s.L - 1) break;
+    t0 = prof.begin();
+    JSSP_HIP(e, launch_hist(a, s.nblk_sel, e->stream));
+    prof.end(kKindHist, t0);
+    t0 = prof.begin();
+    JSSP_HIP(e, launch_partition(a, s.nblk_sel, e->stream));
+    prof.end(kKindPartition, t0);
+    t0 = prof.begin();
+    JSSP_HIP(e, launch_tie(a, e->stream));
+    prof.end(kKindTie, t0);
+    t0 = prof.begin();
+    JSSP_HIP(e, launch_rank(a, order, s.single, s.nblk_rank, e->stream));
+    prof.end(kKindRank, t0);
+  }
+  JSSP_HIP(e, launch_backtrack(a, e->stream));
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  // Results.
+  const size_t hb = 64 + (size_t)s.L * 4 * 3 + 16;
+  rc = ensure_host_staging(e, hb);
+  if (rc) return rc;
+  int* h_res = reinterpret_cast<int*>(e->hbuf);
+  unsigned long long* h_exp = reinterpret_cast<unsigned long long*>(e->hbuf + 32);
+  int* h_seq = reinterpret_cast<int*>(e->hbuf + 64);
+  JSSP_HIP(e, hipMemcpyAsync(h_res, w.result, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipMemcpyAsync(h_exp, w.expanded, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipMemcpyAsync(h_seq, w.seq, (size_t)s.L * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  float ms = 0.f;
+  JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+  res->makespan = h_res[0];
+  res->found = h_res[1];
+  res->expanded_states = (int64_t)*h_exp;
+  res->levels = s.L;
+  res->device_ms = ms;
+  if (res->found && res->sequence) std::memcpy(res->sequence, h_seq, (size_t)s.L * sizeof(int));
+  if (!res->found && res->sequence) { /* untouched, as documented */ }
+  if (e->profiling) {
+    prof.collect();
+    std::vector<int> pc(s.L + 1);
+    std::vector<uint32_t> le(s.L);
+    JSSP_HIP(e, hipMemcpy(pc.data(), w.pcount, (s.L + 1) * sizeof(int), hipMemcpyDeviceToHost));
+    JSSP_HIP(e, hipMemcpy(le.data(), w.lvl_exp, s.L * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    add_algorithmic_bytes(e, s, pc, le);
+  }
+  if (trail_out && s.L > 1)
+    JSSP_HIP(e, hipMemcpy(trail_out, w.trail, (size_t)(s.L - 1) * s.K * sizeof(int), hipMemcpyDeviceToHost));
+  if (pcount_out) JSSP_HIP(e, hipMemcpy(pcount_out, w.pcount, (s.L + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  return JSSP_OK;
+}
+
+// ICH_EXACT / PMS_EXACT: the device expands every child of every parent
+// (cmax, job, machine per erase-order ready position); the host consumes the
+// process-global rand() stream in the reference's order and replays the
+// libstdc++ bounded heap (cpu/common/jssp_types.h:47-58) on order-isomorphic
+// keys; the device then materialises the heap's array in order, which is the
+// parent order of the next level (cpu/v0_ich/solver.cpp:22,42).
+struct HeapEntry {
+  long long ikey;   // ICH: 100*cmax + rand()%100 (orders exactly like the double score)
+  double dkey;      // PMS: cmax/ns + (rand()%1000)/1e6, cpu/v1_pms/solver.cpp:166-167
+  int x;
+};
+struct IchLess {
+  bool operator()(const HeapEntry& a, const HeapEntry& b) const { return a.ikey < b.ikey; }
+};
+struct PmsLess {
+  bool operator()(const HeapEntry& a, const HeapEntry& b) const { return a.dkey < b.dkey; }
+};
+
+template <class Less>
+void heap_insert(std::vector<HeapEntry>& beam, const HeapEntry& h, int k) {
+  if ((int)beam.size() < k) {
+    beam.push_back(h);
+    std::push_heap(beam.begin(), beam.end(), Less{});
+  } else if (Less{}(h, beam.front())) {
+    std::pop_heap(beam.begin(), beam.end(), Less{});
+    beam.back() = h;
+    std::push_heap(beam.begin(), beam.end(), Less{});
+  }
+}
+
+int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
+                int32_t* trail_out, int32_t* pcount_out) {
+  const bool ich = (p->mode == JSSP_MODE_ICH_EXACT);
+  const Shape s = make_shape(in, p);
+  Work w;
+  const size_t need = carve(s, nullptr, w);
+  int rc = ensure_device_arena(e, need);
+  if (rc) return rc;
+  carve(s, e->dbuf, w);
+  rc = upload_ops(e, in, w.ops);
+  if (rc) return rc;
+  const size_t N = (size_t)s.K * s.B;
+  rc = ensure_host_staging(e, N * (sizeof(int) + 2) + (size_t)s.K * 2 * sizeof(int) + 1024);
+  if (rc) return rc;
+  int* h_cmax = reinterpret_cast<int*>(e->hbuf);
+  uint8_t* h_job = reinterpret_cast<uint8_t*>(h_cmax + N);
+  uint8_t* h_mc = h_job + N;
+  int* h_r = reinterpret_cast<int*>(((uintptr_t)(h_mc + N) + 15) & ~(uintptr_t)15);
+  int* h_order = h_r + s.K;
+
+  if (!(p->flags & JSSP_FLAG_GLOBAL_RAND)) srand(p->seed);
+  LevelArgs a = base_args(s, w, p);
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  JSSP_HIP(e, launch_init(a, e->stream));
+  std::vector<HeapEntry> heap;
+  heap.reserve(s.K);
+  int P = 1;
+  long long expanded = 0;
+  int best = JSSP_NO_MAKESPAN, best_p = -1, best_j = -1;
+  int levels = 0;
+  for (int l = 0; l < s.L && P > 0; ++l) {
+    set_level(a, w, l, 0);
+    JSSP_HIP(e, launch_exact_expand(a, P, e->stream));
+    JSSP_HIP(e, hipMemcpyAsync(h_r, w.ex_rcount, (size_t)P * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    JSSP_HIP(e, hipMemcpyAsync(h_cmax, w.ex_cmax, (size_t)P * s.B * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    JSSP_HIP(e, hipMemcpyAsync(h_job, w.ex_job, (size_t)P * s.B, hipMemcpyDeviceToHost, e->stream));
+    if (!ich) JSSP_HIP(e, hipMemcpyAsync(h_mc, w.ex_mc, (size_t)P * s.B, hipMemcpyDeviceToHost, e->stream));
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+    ++levels;
+    heap.clear();
+    const int ns = l + 1;
+    for (int pp = 0; pp < P; ++pp) {
+      const int r = h_r[pp];
+      const size_t base = (size_t)pp * s.B;
+      if (ich) {
+        for (int qq = 0; qq < r; ++qq) {
+          HeapEntry h;
+          h.ikey = 100ll * h_cmax[base + qq] + (rand() % 100);  // cpu/v0_ich/solver.cpp:35
+          h.dkey = 0.0;
+          h.x = (int)(base + qq);
+          ++expanded;
+          heap_insert<IchLess>(heap, h, s.K);
+        }
+      } else {
+        if (r <= 0) continue;
+        const int pivot = rand() % r;                          // cpu/v1_pms/solver.cpp:32
+        const int tmc = h_mc[base + pivot];
+        for (int qq = 0; qq < r; ++qq) {
+          if (h_mc[base + qq] != tmc) continue;
+          const int cm = h_cmax[base + qq];
+          const double noise = (double)(rand() % 1000) / 1000000.0;
+          HeapEntry h;
+          h.ikey = 0;
+          h.dkey = ((double)cm / (double)ns) + noise;
+          h.x = (int)(base + qq);
+          ++expanded;
+          if (cm < p->best_known) heap_insert<PmsLess>(heap, h, s.K);
+          if (ns == s.L && cm < best) { best = cm; best_p = pp; best_j = h_job[base + qq]; }
+        }
+      }
+    }
+    if (l == s.L - 1) {
+      if (ich) {
+        for (const HeapEntry& h : heap) {
+          const int cm = h_cmax[h.x];
+          if (cm < best) { best = cm; best_p = h.x / s.B; best_j = h_job[h.x]; }
+        }
+      }
+      break;
+    }
+    const int count = (int)heap.size();
+    for (int i = 0; i < count; ++i) h_order[i] = heap[i].x;
+    if (count > 0) {
+      JSSP_HIP(e, hipMemcpyAsync(w.ex_order, h_order, (size_t)count * sizeof(int), hipMemcpyHostToDevice, e->stream));
+      JSSP_HIP(e, launch_exact_materialize(a, count, e->stream));
+    }
+    P = count;
+  }
+  const bool found = best < JSSP_NO_MAKESPAN && best_p >= 0;
+  if (found) {
+    a.bt_parent = best_p;
+    a.bt_job = best_j;
+    JSSP_HIP(e, launch_backtrack(a, e->stream));
+  }
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  if (found && res->sequence)
+    JSSP_HIP(e, hipMemcpyAsync(res->sequence, w.seq, (size_t)s.L * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  float ms = 0.f;
+  JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+  res->makespan = best;
+  res->found = found ? 1 : 0;
+  res->expanded_states = expanded;
+  res->levels = levels;
+  res->device_ms = ms;
+  if (trail_out && s.L > 1)
+    JSSP_HIP(e, hipMemcpy(trail_out, w.trail, (size_t)(s.L - 1) * s.K * sizeof(int), hipMemcpyDeviceToHost));
+  if (pcount_out) JSSP_HIP(e, hipMemcpy(pcount_out, w.pcount, (s.L + 1) * sizeof(int), hipMemcpyDeviceToHost));
+  return JSSP_OK;
+}
+
+}  // namespace
+
+// Multi-seed / multi-pass driver (gpu/v1/main.cu:60-71 and
+// cpu/v1_pms/main.cpp:56-73 as one process): every beam of a pass prunes with
+// the bound at the start of the pass; after the pass the bound is the
+// minimum.  Used by jssp_solve for n_seeds > 1 without batching.
+int solve_any(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
+              int32_t* trail_out, int32_t* pcount_out) {
+  int rc = check_instance(e, in);
+  if (rc) return rc;
+  rc = check_params(e, in, p);
+  if (rc) return rc;
+  if (!res) return set_error(e, JSSP_ERR_INVALID, "jssp_solve", "null result");
+  JSSP_HIP(e, hipSetDevice(e->device));
+  res->best_seed_index = 0;
+  if (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT)
+    return solve_exact(e, in, p, res, trail_out, pcount_out);
+  const int seeds = std::max(1, p->n_seeds);
+  const int passes = std::max(1, p->n_passes);
+  if (seeds == 1 && passes == 1)
+    return solve_device(e, in, p, p->seed, p->best_known, res, trail_out, pcount_out);
+  int bound = p->best_known;
+  jssp_result best;
+  std::memset(&best, 0, sizeof(best));
+  best.makespan = bound;
+  std::vector<int32_t> seq((size_t)in->n_jobs * in->n_machines);
+  int64_t expanded = 0;
+  double ms = 0.0;
+  for (int e_ = 0; e_ < passes; ++e_) {
+    const int pass_bound = bound;
+    for (int i = 0; i < seeds; ++i) {
+      jssp_result r;
+      std::memset(&r, 0, sizeof(r));
+      r.sequence = seq.data();
+      rc = solve_device(e, in, p, p->seed + (uint32_t)(e_ * seeds + i), pass_bound, &r, nullptr, nullptr);
+      if (rc) return rc;
+      expanded += r.expanded_states;
+      ms += r.device_ms;
+      if (r.found && r.makespan < bound) {
+        bound = r.makespan;
+        best = r;
+        best.best_seed_index = e_ * seeds + i;
+        if (res->sequence) std::memcpy(res->sequence, seq.data(), seq.size() * sizeof(int32_t));
+      }
+    }
+  }
+  res->makespan = bound;
+  res->found = best.found;
+  res->best_seed_index = best.best_seed_index;
+  res->expanded_states = expanded;
+  res->levels = in->n_jobs * in->n_machines;
+  res->device_ms = ms;
+  return JSSP_OK;
+}
+
+}  // namespace jssp
+
+extern "C" {
+
+int jssp_engine_create(int device, jssp_engine** out) {
+  if (!out) return JSSP_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    return JSSP_ERR_HIP;
+  }
+  if (device < 0 || device >= count) return JSSP_ERR_INVALID;
+  jssp_engine* e = new jssp_engine();
+  e->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev_start) != hipSuccess || hipEventCreate(&e->ev_stop) != hipSuccess) {
+    (void)hipGetLastError();
+    delete e;
+    return JSSP_ERR_HIP;
+  }
+  *out = e;
+  return JSSP_OK;
+}
+
+void jssp_engine_destroy(jssp_engine* e) {
+  if (!e) return;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  for (hipEvent_t ev : e->pev) (void)hipEventDestroy(ev);
+  if (e->ev_start) (void)hipEventDestroy(e->ev_start);
+  if (e->ev_stop) (void)hipEventDestroy(e->ev_stop);
+  if (e->dbuf) (void)hipFree(e->dbuf);
+  if (e->hbuf) (void)hipHostFree(e->hbuf);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+const char* jssp_last_error(const jssp_engine* e) { return e ? e->err.c_str() : "null engine"; }
+
+int jssp_solve(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res) {
+  if (!e) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  return jssp::solve_any(e, in, p, res, nullptr, nullptr);
+}
+
+int jssp_solve_trace(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
+                     int32_t* trail, int32_t* pcount) {
+  if (!e) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  if (p && (p->n_seeds > 1 || p->n_passes > 1))
+    return jssp::set_error(e, JSSP_ERR_INVALID, "jssp_solve_trace", "trace needs a single beam");
+  return jssp::solve_any(e, in, p, res, trail, pcount);
+}
+
+int jssp_engine_set_profiling(jssp_engine* e, int32_t on) {
+  if (!e) return JSSP_ERR_INVALID;
+  e->profiling = on != 0;
+  for (auto& k : e->kt) k = KernelTime();
+  return JSSP_OK;
+}
+
+int jssp_engine_kernel_time(const jssp_engine* e, int32_t kind, double* total_ms, int64_t* launches,
+                            double* bytes) {
+  if (!e || kind < 0 || kind >= kNumKinds) return JSSP_ERR_INVALID;
+  if (total_ms) *total_ms = e->kt[kind].ms;
+  if (launches) *launches = e->kt[kind].launches;
+  if (bytes) *bytes = e->kt[kind].bytes;
+  return JSSP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// engine.hpp -- internal state of a jssp_engine, shared by the solve, batch
+// and serving translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/jssp.h"
+
+// Kernel kinds timed by the optional profiler.
+enum { kKindScore = 0, kKindHist = 1, kKindPartition = 2, kKindTie = 3, kKindRank = 4, kNumKinds = 5 };
+
+struct KernelTime {
+  double ms = 0.0;
+  int64_t launches = 0;
+  double bytes = 0.0;
+};
+
+struct jssp_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+  char* dbuf = nullptr;   // device arena (grows, reused across solves)
+  size_t dcap = 0;
+  char* hbuf = nullptr;   // pinned host staging (grows)
+  size_t hcap = 0;
+  std::string err;
+  bool profiling = false;
+  std::vector<hipEvent_t> pev;  // event pool for per-kernel timing
+  KernelTime kt[kNumKinds];
+  std::recursive_mutex mu;
+};
+
+namespace jssp {
+
+// Bump allocator used twice: a dry run (base == nullptr) to size a plan,
+// then the real carve over the arena.
+struct Bump {
+  char* base = nullptr;
+  size_t off = 0;
+  template <class T>
+  T* take(size_t count) {
+    off = (off + 255) & ~(size_t)255;
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * sizeof(T) + 0;
+    return p;
+  }
+};
+
+int set_error(jssp_engine* e, int code, const char* where, const std::string& msg);
+int ensure_device_arena(jssp_engine* e, size_t bytes);
+int ensure_host_staging(jssp_engine* e, size_t bytes);
+int check_instance(jssp_engine* e, const jssp_instance* in);
+int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p);
+
+}  // namespace jssp
+
+#define JSSP_HIP(e, call)                                                              \
+  do {                                                                                 \
+    hipError_t _r = (call);                                                            \
+    if (_r != hipSuccess)                                                              \
+      return ::jssp::set_error((e), JSSP_ERR_HIP, #call, hipGetErrorString(_r));       \
+  } while (0)

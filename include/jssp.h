@@ -1,0 +1,192 @@
+/*
+ * jssp.h -- C ABI of the MI355X beam-search engine for job-shop scheduling.
+ *
+ * Plain C types only (no HIP or torch types), so cgo / JNI / ctypes / N-API
+ * can bind it directly.  Every function returns a jssp_status (0 = OK) unless
+ * stated otherwise; no function calls exit().  The message of the last error
+ * on an engine is jssp_last_error(engine).
+ *
+ * Reference interfaces replaced (the reference has no FFI layer; drivers link
+ * the solver sources directly, reference Makefile:55-93):
+ *   beam_search_ich         cpu/v0_ich/solver.h:6      -> jssp_solve, mode ICH_EXACT
+ *   beam_search_pms         cpu/v1_pms/solver.h:6      -> jssp_solve, mode PMS_EXACT
+ *   run_gpu_beam_search_v1  gpu/v1/solver.cuh:6-12     -> jssp_solve, mode PMS_GPU, phases 1
+ *   run_gpu_beam_search_v2  gpu/v2_twophases/solver.cuh:6-12 -> jssp_solve, mode PMS_GPU, phases 2
+ *   parse_instance          cpu/common/instance_parser.h:9   -> jssp_parse_instance_file
+ *   resolve_instance_file   cpu/common/instance_parser.h:6   -> jssp_resolve_instance_file
+ *   drivers' multi-pass loop + MPI_Allreduce(MIN)  gpu/v1/main.cu:60-71,
+ *                           cpu/v1_pms/main.cpp:56-73  -> jssp_solve with n_seeds/n_passes
+ * C++ callers that want the reference's exact function signatures include
+ * jssp_compat.hpp instead.
+ */
+#ifndef JSSP_H_
+#define JSSP_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* "No makespan found" sentinel of the reference (cpu/common/jssp_types.h:33,
+ * gpu/v1/solver.cu:277). */
+#define JSSP_NO_MAKESPAN 500000
+
+/* Limits of this engine (the reference's are n <= 100, m <= 31,
+ * n*m + 2 <= 2100, gpu/common/gpu_common.cuh:20-22). */
+#define JSSP_MAX_JOBS 128
+#define JSSP_MAX_MACHINES 255
+#define JSSP_MAX_BEAM 65536
+
+typedef enum jssp_status {
+  JSSP_OK = 0,
+  JSSP_ERR_INVALID = 1,   /* bad argument or malformed instance */
+  JSSP_ERR_LIMIT = 2,     /* beyond JSSP_MAX_* */
+  JSSP_ERR_HIP = 3,       /* HIP runtime error (no device, launch failure ...) */
+  JSSP_ERR_NOMEM = 4,     /* device or host allocation failed */
+  JSSP_ERR_IO = 5,        /* file not found / unreadable */
+  JSSP_ERR_BUDGET = 6,    /* request larger than the server's HBM budget */
+  JSSP_ERR_PENDING = 7,   /* serving: result not ready yet */
+  JSSP_ERR_NOTFOUND = 8,  /* serving: unknown ticket */
+  JSSP_ERR_CANCELLED = 9  /* serving: request cancelled or timed out */
+} jssp_status;
+
+typedef enum jssp_mode {
+  /* cpu/v0_ich/solver.cpp:8-54 bit-exact: operation-level branching, score
+   * cmax + (rand()%100)*0.01 from the process-global rand(), bounded
+   * max-heap selection.  The GPU expands every child; the host replays the
+   * rand() stream and the heap. */
+  JSSP_MODE_ICH_EXACT = 0,
+  /* Operation-level branching with the GPU reference's FP32 score and hash
+   * noise, ready list in erase-then-append order; fully on the device. */
+  JSSP_MODE_ICH_DET = 1,
+  /* gpu/v1/solver.cu:9-157 semantics: hash-chosen pivot machine, ready list
+   * in swap-remove order, FP32 score, survivors sorted by (score, child). */
+  JSSP_MODE_PMS_GPU = 2,
+  /* cpu/v1_pms/solver.cpp:8-70 bit-exact (rand() pivot and noise, FP64
+   * normalised score, prune vs best_known); host replays rand() and heap. */
+  JSSP_MODE_PMS_EXACT = 3
+} jssp_mode;
+
+/* Flags for jssp_params.flags */
+#define JSSP_FLAG_GLOBAL_RAND 0x1u  /* exact modes: do not srand(seed); continue the process-global rand() stream */
+
+typedef struct jssp_instance {
+  int32_t n_jobs;
+  int32_t n_machines;
+  const int32_t* machine;   /* [n_jobs * n_machines], 0-based machine of op k of job j at j*m + k */
+  const int32_t* duration;  /* [n_jobs * n_machines] */
+} jssp_instance;
+
+typedef struct jssp_params {
+  int32_t beam_width;   /* K >= 1 */
+  int32_t mode;         /* jssp_mode */
+  int32_t phases;       /* 1: materialise every child, then select (gpu/v1);
+                           2: score only, select, materialise the K survivors (gpu/v2) */
+  int32_t best_known;   /* prune bound; JSSP_NO_MAKESPAN for none */
+  uint32_t seed;        /* GPU modes: random_seed of the reference (level seed = seed + level);
+                           exact modes: srand() seed unless JSSP_FLAG_GLOBAL_RAND */
+  int32_t n_seeds;      /* PMS_GPU / ICH_DET: independent beams with seeds seed, seed+1, ...
+                           sharing one device best bound (0 or 1 = single beam) */
+  int32_t n_passes;     /* multi-seed: passes; pass e prunes with the bound after pass e-1 (0 = 1) */
+  uint32_t flags;
+} jssp_params;
+
+typedef struct jssp_result {
+  int32_t makespan;         /* PMS_GPU / ICH_DET: min(best_known, best found) as gpu/v1/solver.cu:277-297;
+                               ICH_EXACT: best cmax of the final beam; PMS_EXACT: best complete cmax */
+  int32_t found;            /* 1 when `sequence` was filled */
+  int32_t* sequence;        /* caller-owned [n*m] or NULL: 1-based reference op ids (j*m + k + 1) */
+  int64_t expanded_states;  /* children evaluated, summed over levels */
+  int32_t levels;           /* levels run */
+  int32_t best_seed_index;  /* multi-seed: which beam produced the result */
+  double device_ms;         /* wall time of the solve on the engine's stream (HIP events) */
+} jssp_result;
+
+typedef struct jssp_engine jssp_engine;
+
+/* Engine: owns one HIP stream, a growable device arena and the tuning table.
+ * One engine per host thread (engines are independent). */
+int jssp_engine_create(int device, jssp_engine** out);
+void jssp_engine_destroy(jssp_engine* engine);
+const char* jssp_last_error(const jssp_engine* engine);
+
+/* One solve.  Inputs are host pointers; they are uploaded inside the call. */
+int jssp_solve(jssp_engine* engine, const jssp_instance* inst, const jssp_params* params,
+               jssp_result* result);
+
+/* As jssp_solve, and also returns the per-level survivor trail
+ * (trail[l*K + q] = (parent << 8) | job for survivor q after level l,
+ * (n*m - 1) * K entries) and pcount[l] (n*m + 1 entries).  Either may be NULL. */
+int jssp_solve_trace(jssp_engine* engine, const jssp_instance* inst, const jssp_params* params,
+                     jssp_result* result, int32_t* trail, int32_t* pcount);
+
+/* Batched solve: advances all `count` instances together, one fused level
+ * kernel per step replayed from a captured hipGraph.  Results equal those of
+ * jssp_solve on each instance alone.  Modes ICH_DET and PMS_GPU. */
+int jssp_solve_batch(jssp_engine* engine, int32_t count, const jssp_instance* insts,
+                     const jssp_params* params, jssp_result* results);
+
+/* Per-kernel timing accumulated over the solves run since profiling was
+ * switched on (jssp_engine_set_profiling(engine, 1) also resets it), for
+ * roofline reporting.  kind: 0 = expand/score, 1 = selection histogram,
+ * 2 = partition, 3 = threshold-bin sort, 4 = rank + materialise/gather.
+ * total_ms is the sum of HIP-event durations around each launch on the
+ * engine's stream; algorithmic_bytes the bytes the algorithm must move. */
+int jssp_engine_set_profiling(jssp_engine* engine, int32_t on);
+int jssp_engine_kernel_time(const jssp_engine* engine, int32_t kind, double* total_ms,
+                            int64_t* launches, double* algorithmic_bytes);
+
+/* ---- serving: continuous batching with HBM-budget admission ------------ */
+typedef struct jssp_server jssp_server;
+typedef struct jssp_server_config {
+  int64_t hbm_budget_bytes;  /* admission budget for resident beams */
+  int32_t max_slots;         /* concurrently resident requests */
+  int32_t reserved;
+} jssp_server_config;
+
+int jssp_server_create(jssp_engine* engine, const jssp_server_config* cfg, jssp_server** out);
+void jssp_server_destroy(jssp_server* server);
+/* Queues a request; inputs are copied.  *ticket identifies it. */
+int jssp_serve_submit(jssp_server* server, const jssp_instance* inst, const jssp_params* params,
+                      uint64_t* ticket);
+/* Advances the device by up to `levels` level steps, admitting queued work as
+ * slots and budget free up.  Returns the number of active requests in *active. */
+int jssp_serve_step(jssp_server* server, int32_t levels, int32_t* active);
+/* JSSP_OK and the result when done, JSSP_ERR_PENDING otherwise.  The result's
+ * sequence pointer is caller-owned as in jssp_solve. */
+int jssp_serve_poll(jssp_server* server, uint64_t ticket, jssp_result* result);
+int jssp_serve_cancel(jssp_server* server, uint64_t ticket);
+/* Runs until every submitted request is finished. */
+int jssp_serve_drain(jssp_server* server);
+/* Submission / admission / completion times (seconds, steady clock) of a ticket. */
+int jssp_serve_times(jssp_server* server, uint64_t ticket, double* submit_s, double* admit_s,
+                     double* done_s);
+
+/* ---- host-side instance I/O and checks (no device needed) -------------- */
+/* resolve_instance_file (cpu/common/instance_parser.cpp:8-34): a direct path,
+ * then ./laXX.txt, ./taXX.txt, {.,..,../..}/lawrence_instances/laXX.txt,
+ * {.,..,../..}/taillard_instances/taXX.  Returns 1 if found. */
+int jssp_resolve_instance_file(const char* arg, char* out_path, int32_t path_size);
+/* parse_instance (cpu/common/instance_parser.cpp:37-93): header "n m", then
+ * n rows of m (machine, duration) pairs, 0-based machines.  Fills up to `cap`
+ * ops.  A truncated body is an error here (the reference silently reads
+ * machine 0 / duration 0). */
+int jssp_parse_instance_file(const char* path, int32_t* n_jobs, int32_t* n_machines,
+                             int32_t* machine, int32_t* duration, int32_t cap);
+/* Seeded synthetic instance: per job a Fisher-Yates machine order, durations
+ * uniform in [dmin, dmax] (splitmix64 stream). */
+int jssp_generate_instance(uint64_t seed, int32_t n_jobs, int32_t n_machines, int32_t dmin,
+                           int32_t dmax, int32_t* machine, int32_t* duration);
+/* Re-simulates a sequence of 1-based op ids; *makespan = -1 if infeasible. */
+int jssp_validate_sequence(const jssp_instance* inst, const int32_t* sequence, int32_t* makespan);
+/* max(max job length, max machine load) */
+int32_t jssp_lower_bound(const jssp_instance* inst);
+
+/* Library version string. */
+const char* jssp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JSSP_H_ */

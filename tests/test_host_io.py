@@ -1,0 +1,99 @@
+"""Host-side logic of the library (no device): instance files, the resolver,
+the generator, the validator and the lower bound."""
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import beamsearch_jssp_gpu_amd as bs
+from testlib import GOLDEN, load_json, lower_bound, orc_simulate, splitmix_instance
+
+
+def test_parser_matches_reference_parser_on_lawrence():
+    law = load_json("lawrence.json")
+    inst = bs.parse_instance(os.path.join(GOLDEN, "instances", "la01.txt"))
+    assert (inst.nbjob, inst.nbmachine, inst.nboperation) == (10, 5, 52)
+    assert inst.machine.ravel().tolist() == law["la01"]["machine"]
+    assert inst.duration.ravel().tolist() == law["la01"]["duration"]
+
+
+def test_parser_taillard_formats():
+    a = bs.parse_instance(os.path.join(GOLDEN, "instances", "ta01"))
+    assert a.machine.shape == (15, 15)
+    b = bs.parse_instance(os.path.join(GOLDEN, "instances", "ta71.txt"))
+    assert b.machine.shape == (100, 20)
+    for inst in (a, b):
+        for row in inst.machine:
+            assert sorted(row.tolist()) == list(range(inst.nbmachine))
+
+
+def test_parser_errors(tmp_path):
+    with pytest.raises(bs.JsspError) as e:
+        bs.parse_instance(str(tmp_path / "missing.txt"))
+    assert e.value.code == 5
+    bad = tmp_path / "trunc.txt"
+    bad.write_text("3 2\n0 5 1 6\n1 4\n")
+    with pytest.raises(bs.JsspError) as e:
+        bs.parse_instance(str(bad))
+    assert e.value.code == 1
+    hdr = tmp_path / "hdr.txt"
+    hdr.write_text("x y\n")
+    with pytest.raises(bs.JsspError):
+        bs.parse_instance(str(hdr))
+
+
+def test_resolver_search_order(tmp_path, monkeypatch):
+    """cpu/common/instance_parser.cpp:8-34: a direct path first, then
+    ./laXX.txt, ./taXX.txt, then the lawrence/taillard folders."""
+    work = tmp_path / "a" / "b"
+    work.mkdir(parents=True)
+    (tmp_path / "lawrence_instances").mkdir()
+    (tmp_path / "taillard_instances").mkdir()
+    src = os.path.join(GOLDEN, "instances", "la01.txt")
+    shutil.copy(src, tmp_path / "lawrence_instances" / "la07.txt")
+    shutil.copy(os.path.join(GOLDEN, "instances", "ta01"), tmp_path / "taillard_instances" / "ta07")
+    shutil.copy(os.path.join(GOLDEN, "instances", "ta01"), tmp_path / "taillard_instances" / "ta09")
+    monkeypatch.chdir(work)
+    # Lawrence wins over Taillard for the same id (../../lawrence before ../../taillard)
+    assert bs.resolve_instance_file("07") == "../../lawrence_instances/la07.txt"
+    assert bs.resolve_instance_file("09") == "../../taillard_instances/ta09"
+    shutil.copy(src, work / "la07.txt")
+    assert bs.resolve_instance_file("07") == "./la07.txt"
+    assert bs.resolve_instance_file(src) == src
+    assert bs.resolve_instance_file("99") is None
+
+
+def test_generator_matches_fixture_stream():
+    g = load_json("golden_ref.json")["randoms"]
+    for name, d in g.items():
+        inst = bs.generate_instance(d["seed"], d["n"], d["m"], d["dmin"], d["dmax"])
+        assert inst.machine.ravel().tolist() == d["machine"], name
+        assert inst.duration.ravel().tolist() == d["duration"], name
+    big = bs.generate_instance(1, 50, 20, 1, 99)
+    ref = splitmix_instance(1, 50, 20, 1, 99)
+    assert np.array_equal(big.machine, ref.machine) and np.array_equal(big.duration, ref.duration)
+    assert big.duration.min() >= 1 and big.duration.max() <= 99
+    for row in big.machine:
+        assert sorted(row.tolist()) == list(range(20))
+
+
+def test_validator_and_lower_bound():
+    inst = bs.generate_instance(3, 6, 4, 1, 20)
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        # random feasible sequence
+        left = [4] * 6
+        nxt = [0] * 6
+        seq = []
+        while len(seq) < 24:
+            j = int(rng.choice([k for k in range(6) if left[k]]))
+            seq.append(j * 4 + nxt[j] + 1)
+            nxt[j] += 1
+            left[j] -= 1
+        ms = bs.validate_sequence(inst, np.array(seq))
+        assert ms == orc_simulate(inst, np.array(seq, np.int32))
+        assert ms >= bs.lower_bound(inst)
+    bad = np.arange(24, 0, -1, dtype=np.int32)
+    assert bs.validate_sequence(inst, bad) == -1
+    assert bs.lower_bound(inst) == lower_bound(inst)

