@@ -31,6 +31,8 @@ __all__ = [
 NO_MAKESPAN = 500000
 MODE_ICH_EXACT, MODE_ICH_DET, MODE_PMS_GPU, MODE_PMS_EXACT = 0, 1, 2, 3
 FLAG_GLOBAL_RAND = 0x1
+FLAG_RADIX_CUT = 0x2
+FLAG_GENERIC_DIGIT = 0x4
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -86,6 +88,8 @@ _SIGNATURES = {
     "jssp_engine_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "jssp_engine_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
+    "jssp_engine_level_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32),
+                                               ctypes.c_int32]),
     "jssp_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_ServerConfig),
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "jssp_server_destroy": (None, [ctypes.c_void_p]),
@@ -244,6 +248,12 @@ class Engine:
 
     def set_profiling(self, on: bool):
         self._check(self.lib.jssp_engine_set_profiling(self.h, 1 if on else 0))
+
+    def level_stats(self, level: int) -> dict:
+        out = (ctypes.c_uint32 * 10)()
+        self._check(self.lib.jssp_engine_level_stats(self.h, level, out, 10))
+        keys = ["kept", "survivors", "lo", "shift", "T1", "below", "tie", "need", "sure_ctr", "tie_ctr"]
+        return dict(zip(keys, list(out)))
 
     def kernel_time(self, kind: int):
         ms, n, b = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -104,16 +105,19 @@ struct Work {
   int* cbuf;
   unsigned long long* keys;
   uint8_t* cjob;
-  unsigned long long* bucketed;
+  unsigned long long* cand;
+  unsigned long long* ref_buf;
+  uint16_t* kdig;
+  uint32_t* tbits;
   uint32_t* hist[2];
-  uint32_t* fill[2];
-  uint32_t* excl;
   BlockStat* bstat;
-  SelState* sel;
+  RankStat* rstat;
+  SelState* sel[2];
   int* pcount;
   int* trail;
   unsigned long long* expanded;
   uint32_t* lvl_exp;
+  SelState* sel_log;
   int* seq;
   int* result;
   int* ex_cmax;
@@ -126,7 +130,7 @@ struct Work {
 struct Shape {
   int n, m, S, B, K, L;
   bool single, exact;
-  int nblk_score, nblk_sel, nblk_rank;
+  int nblk_score, nblk_filter, nblk_rank;
 };
 
 Shape make_shape(const jssp_instance* in, const jssp_params* p) {
@@ -140,8 +144,8 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p) {
   s.single = !s.exact && p->phases == 1;
   const long long N = (long long)s.K * s.B;
   s.nblk_score = (int)std::max(1, std::min((s.K + kScoreWaves - 1) / kScoreWaves, 512));
-  s.nblk_sel = (int)std::max(1ll, std::min((N + 4095) / 4096, 256ll));
-  s.nblk_rank = (int)std::max(1, std::min((s.K + kRankWaves - 1) / kRankWaves, 2048));
+  s.nblk_filter = (int)std::max(1ll, std::min((N + 4095) / 4096, 256ll));
+  s.nblk_rank = (int)std::max(1, std::min((s.K + kRankWaves - 1) / kRankWaves, 256));
   return s;
 }
 
@@ -155,18 +159,21 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.cbuf = s.single ? b.take<int>(N * s.S) : nullptr;
   w.keys = b.take<unsigned long long>(std::max(N, (size_t)2 * s.K));
   w.cjob = b.take<uint8_t>(N);
-  w.bucketed = b.take<unsigned long long>(N);
+  w.cand = b.take<unsigned long long>(N);
+  w.ref_buf = b.take<unsigned long long>(N + s.K);
+  w.kdig = b.take<uint16_t>(N);
+  w.tbits = b.take<uint32_t>((N + 31) / 32 + 2);
   w.hist[0] = b.take<uint32_t>(kNB);
   w.hist[1] = b.take<uint32_t>(kNB);
-  w.fill[0] = b.take<uint32_t>(kNB);
-  w.fill[1] = b.take<uint32_t>(kNB);
-  w.excl = b.take<uint32_t>(kNB);
   w.bstat = b.take<BlockStat>(s.nblk_score);
-  w.sel = b.take<SelState>(1);
+  w.rstat = b.take<RankStat>(s.nblk_rank);
+  w.sel[0] = b.take<SelState>(1);
+  w.sel[1] = b.take<SelState>(1);
   w.pcount = b.take<int>(s.L + 1);
   w.trail = b.take<int>((size_t)std::max(1, s.L - 1) * s.K);
   w.expanded = b.take<unsigned long long>(1);
   w.lvl_exp = b.take<uint32_t>(s.L);
+  w.sel_log = b.take<SelState>(s.L);
   w.seq = b.take<int>(s.L);
   w.result = b.take<int>(4);
   if (s.exact) {
@@ -181,6 +188,27 @@ size_t carve(const Shape& s, char* base, Work& w) {
   return b.off + 256;
 }
 
+}  // namespace
+
+// The exact selection digit is valid when, at every level, a lower makespan
+// always gives a lower FP32 score: for cmax c1 < c2 at ns scheduled ops,
+// score(c1) <= c1/ns + 1e-6 + ulp and score(c2) >= c2/ns - ulp/2, so it
+// suffices that 1/L > 1e-6 + 2 ulp(2 dmax + 1), since every score is below
+// dmax + 1e-6 (a semi-active schedule grows cmax by at most one duration per
+// op, so cmax/ns <= dmax).
+bool exact_digit_applies(const jssp_instance* in) {
+  int dmax = 0;
+  for (int i = 0; i < in->n_jobs * in->n_machines; ++i) dmax = std::max(dmax, (int)in->duration[i]);
+  if (dmax <= 0) return false;
+  int e = 0;
+  std::frexp(2.0 * dmax + 1.0, &e);  // 2 dmax + 1 in [2^(e-1), 2^e)
+  const double ulp = std::ldexp(1.0, (e - 1) - 23);
+  const double L = (double)in->n_jobs * in->n_machines;
+  return 1.0 / L > 1e-6 + 2.0 * ulp;
+}
+
+namespace {
+
 LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   LevelArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -190,16 +218,22 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.cbuf = w.cbuf;
   a.keys = w.keys;
   a.cjob = w.cjob;
-  a.bucketed = w.bucketed;
-  a.hist = w.hist[0]; a.fill = w.fill[0];
-  a.hist_next = w.hist[1]; a.fill_next = w.fill[1];
-  a.excl = w.excl;
+  a.cand = w.cand;
+  a.ref_buf = w.ref_buf;
+  a.kdig = w.kdig;
+  a.tbits = w.tbits;
+  a.hist = w.hist[0];
+  a.hist_next = w.hist[1];
   a.bstat = w.bstat;
-  a.sel = w.sel;
+  a.rstat = w.rstat;
+  a.sel = w.sel[0];
+  a.sel_prev = w.sel[1];
+  a.sel_next = w.sel[1];
   a.pcount = w.pcount;
   a.trail = w.trail;
   a.expanded = w.expanded;
   a.lvl_exp = w.lvl_exp;
+  a.sel_log = nullptr;
   a.seq_out = w.seq;
   a.result = w.result;
   a.ex_cmax = w.ex_cmax; a.ex_job = w.ex_job; a.ex_mc = w.ex_mc;
@@ -208,6 +242,9 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.level = 0;
   a.best_known = p->best_known;
   a.nblk_score = s.nblk_score;
+  a.nblk_rank = s.nblk_rank;
+  a.bm_words = (p->flags & JSSP_FLAG_RADIX_CUT) ? 0u
+               : (uint32_t)std::min<size_t>(((size_t)s.K * s.B + 31) / 32, kBitmapWordsMax);
   a.lseed = p->seed;
   a.bt_parent = -1; a.bt_job = -1;
   return a;
@@ -219,9 +256,10 @@ void set_level(LevelArgs& a, const Work& w, int level, uint32_t seed) {
   a.beam_cur = w.beam[level & 1];
   a.beam_next = w.beam[(level + 1) & 1];
   a.hist = w.hist[level & 1];
-  a.fill = w.fill[level & 1];
   a.hist_next = w.hist[(level + 1) & 1];
-  a.fill_next = w.fill[(level + 1) & 1];
+  a.sel = w.sel[level & 1];
+  a.sel_prev = w.sel[(level + 1) & 1];
+  a.sel_next = w.sel[(level + 1) & 1];
 }
 
 // Per-kernel HIP-event timing (optional).
@@ -277,9 +315,9 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
                            const std::vector<uint32_t>& lexp) {
   // Algorithmic bytes per launch: the parent snapshots read (4S each), the
   // children written (4S each, single-phase) and the 8-byte (score, child)
-  // keys; selection reads every key once per pass and writes the survivors;
-  // materialisation reads a parent and writes a child snapshot, the trail
-  // entry and reads the survivor key.
+  // keys; the filter reads every key once and writes the survivors;
+  // materialisation reads a parent and writes a child snapshot and the trail
+  // entry, and reads the survivor key.
   const double S4 = 4.0 * s.S;
   for (int l = 0; l < s.L; ++l) {
     const double P = pc[l], C = lexp[l];
@@ -287,9 +325,7 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
     if (s.single) e->kt[kKindScore].bytes += S4 * P + (S4 + 8.0) * C;
     else e->kt[kKindScore].bytes += S4 * P + 8.0 * C;
     if (l == s.L - 1) break;
-    e->kt[kKindHist].bytes += 8.0 * C;
-    e->kt[kKindPartition].bytes += 8.0 * C + 8.0 * Kp;
-    // the threshold-bin sort moves a few KB; its bytes are not tracked
+    e->kt[kKindFilter].bytes += 8.0 * C + 8.0 * Kp;
     e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 12.0 * Kp;
   }
 }
@@ -309,6 +345,8 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   rc = upload_ops(e, in, w.ops);
   if (rc) return rc;
   LevelArgs a = base_args(s, w, &q);
+  a.exact_digit = (!(q.flags & JSSP_FLAG_GENERIC_DIGIT) && exact_digit_applies(in)) ? 1 : 0;
+  if (e->profiling) a.sel_log = w.sel_log;
   const int mode = (q.mode == JSSP_MODE_ICH_DET) ? 1 : 2;
   const int order = (mode == 1) ? kOrderErase : kOrderSwap;
   Prof prof{e, {}, 0};
@@ -321,16 +359,10 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
     prof.end(kKindScore, t0);
     if (l == s.L - 1) break;
     t0 = prof.begin();
-    JSSP_HIP(e, launch_hist(a, s.nblk_sel, e->stream));
-    prof.end(kKindHist, t0);
+    JSSP_HIP(e, launch_filter(a, s.nblk_filter, e->stream));
+    prof.end(kKindFilter, t0);
     t0 = prof.begin();
-    JSSP_HIP(e, launch_partition(a, s.nblk_sel, e->stream));
-    prof.end(kKindPartition, t0);
-    t0 = prof.begin();
-    JSSP_HIP(e, launch_tie(a, e->stream));
-    prof.end(kKindTie, t0);
-    t0 = prof.begin();
-    JSSP_HIP(e, launch_rank(a, order, s.single, s.nblk_rank, e->stream));
+    JSSP_HIP(e, launch_rank(a, order, s.single, e->stream));
     prof.end(kKindRank, t0);
   }
   JSSP_HIP(e, launch_backtrack(a, e->stream));
@@ -362,6 +394,8 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
     JSSP_HIP(e, hipMemcpy(pc.data(), w.pcount, (s.L + 1) * sizeof(int), hipMemcpyDeviceToHost));
     JSSP_HIP(e, hipMemcpy(le.data(), w.lvl_exp, s.L * sizeof(uint32_t), hipMemcpyDeviceToHost));
     add_algorithmic_bytes(e, s, pc, le);
+    e->level_log.resize((size_t)s.L);
+    JSSP_HIP(e, hipMemcpy(e->level_log.data(), w.sel_log, (size_t)s.L * sizeof(SelState), hipMemcpyDeviceToHost));
   }
   if (trail_out && s.L > 1)
     JSSP_HIP(e, hipMemcpy(trail_out, w.trail, (size_t)(s.L - 1) * s.K * sizeof(int), hipMemcpyDeviceToHost));
@@ -624,6 +658,14 @@ int jssp_engine_set_profiling(jssp_engine* e, int32_t on) {
   if (!e) return JSSP_ERR_INVALID;
   e->profiling = on != 0;
   for (auto& k : e->kt) k = KernelTime();
+  return JSSP_OK;
+}
+
+int jssp_engine_level_stats(const jssp_engine* e, int32_t level, uint32_t* out, int32_t count) {
+  if (!e || !out || level < 0 || (size_t)level >= e->level_log.size()) return JSSP_ERR_INVALID;
+  const jssp::SelState& st = e->level_log[(size_t)level];
+  const uint32_t v[10] = {st.V, st.Kp, st.lo, st.s1, st.T1, st.cum_lt, st.t1, st.need, st.sure_ctr, st.tie_ctr};
+  for (int i = 0; i < count && i < 10; ++i) out[i] = v[i];
   return JSSP_OK;
 }
 

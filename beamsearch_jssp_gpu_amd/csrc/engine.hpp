@@ -9,9 +9,10 @@
 #include <vector>
 
 #include "../../include/jssp.h"
+#include "level_kernels.hpp"
 
 // Kernel kinds timed by the optional profiler.
-enum { kKindScore = 0, kKindHist = 1, kKindPartition = 2, kKindTie = 3, kKindRank = 4, kNumKinds = 5 };
+enum { kKindScore = 0, kKindFilter = 1, kKindRank = 2, kNumKinds = 3 };
 
 struct KernelTime {
   double ms = 0.0;
@@ -31,6 +32,7 @@ struct jssp_engine {
   bool profiling = false;
   std::vector<hipEvent_t> pev;  // event pool for per-kernel timing
   KernelTime kt[kNumKinds];
+  std::vector<jssp::SelState> level_log;  // per-level selection state of the last profiled solve
   std::recursive_mutex mu;
 };
 

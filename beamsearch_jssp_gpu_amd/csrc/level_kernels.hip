@@ -6,24 +6,35 @@
 // parent, one thread per branch, 17 KB of private arrays per thread,
 // gpu/v1/solver.cu:9-157, gpu/v2_twophases/solver.cu:9-147), a full CUB radix
 // sort of every slot (gpu/v1/solver.cu:255) and a one-thread-per-survivor copy
-// (gpu/v1/solver.cu:160-176, gpu/v2_twophases/solver.cu:150-254).  Here:
-//   * one wave per parent; lane j holds job j, so the ready list, the pivot
-//     filter and the child slots come from ballots and popcounts, and every
-//     child's start/finish/cmax is computed in registers;
-//   * children are written as compact snapshots with lanes covering word
-//     offsets (coalesced), only for valid children;
-//   * top-K is a radix select: one histogram of a data-ranged 11-bit digit of
-//     the score bits, one partition of the keys at or below the threshold bin
-//     into per-bin buckets, an LDS sort of the threshold bin, and a rank by
-//     counting inside each (small) bucket.  The output is the survivors in
-//     exact (score bits, child index) order -- the order CUB's stable sort
-//     gives the reference -- and the rank is the survivor's slot in the
-//     next beam;
-//   * survivors are rebuilt from their parent (two-phase) or copied from the
-//     child buffer (single-phase) by one wave per survivor.
-// All counts (parents per level, survivors) stay on the device, so the host
-// enqueues a fixed sequence of launches with no readback.
+// (gpu/v1/solver.cu:160-176, gpu/v2_twophases/solver.cu:150-254).  Here a level
+// is three launches:
+//   k_score   one wave per parent; lane j holds job j, so the ready list, the
+//             pivot filter and the child slots come from ballots/popcounts and
+//             every child's start/finish/cmax is computed in registers.  Each
+//             kept child gets the 64-bit key (score bits << 32 | child index)
+//             and is counted in an LDS histogram of its first selection digit
+//             (kNB bins of the score bits above a proven lower bound), merged
+//             into HBM once per block.  Single-phase also writes the child's
+//             compact snapshot (lanes cover word offsets: coalesced stores).
+//   k_filter  every block scans the histogram (LDS, wave prefix) to find the
+//             bin holding the K-th key, and compacts the keys below it with
+//             block-aggregated offsets.  When that bin holds one score value
+//             (the usual case with the exact digit), its keys are only marked
+//             in a bitmap over child indices, by ballot, and the last block to
+//             finish takes the first `need` set bits -- the ties in index
+//             order.  Otherwise the bin's keys are compacted too and the last
+//             block cuts them by exact score histograms plus the bitmap.
+//   k_rank    each block stages the K' survivor keys in LDS; one wave per
+//             survivor counts the smaller keys (its rank = its slot in the
+//             next beam, i.e. exact (score bits, child index) order -- the
+//             order CUB's stable sort gives the reference) and materialises
+//             the child from its parent (two-phase) or copies it from the
+//             child buffer (single-phase).
+// Every count stays on the device, so the host enqueues a fixed sequence of
+// launches with no readback, and the sequence is graph-capturable.
 #include <hip/hip_runtime.h>
+
+#include <climits>
 
 #include "level_kernels.hpp"
 #include "snapshot.hpp"
@@ -38,7 +49,12 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
   return v;
 }
-__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o));
   return v;
@@ -56,6 +72,14 @@ __device__ __forceinline__ u64 wave_min64(u64 v) {
   }
   return v;
 }
+__device__ __forceinline__ u64 wave_max64(u64 v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    u64 w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -64,9 +88,62 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
   }
   return v;
 }
+// Set bits of `mask` below this lane.
+__device__ __forceinline__ uint32_t lanes_below(u64 mask, int lane) {
+  return (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
 
-__device__ __forceinline__ uint32_t sel_digit(u64 key, uint32_t lo, uint32_t shift) {
-  return ((uint32_t)(key >> 32) - lo) >> shift;
+// Streams src[0, cnt) through one block with 8 independent loads in flight
+// per lane; f(key, valid) is called by every lane of every wave (valid is
+// false past the end) so f may use wave-wide ballots.
+template <class F>
+__device__ __forceinline__ void for_keys(const u64* src, uint32_t cnt, F f) {
+  constexpr int U = 8;
+  const uint32_t bd = blockDim.x;
+  for (uint32_t base = 0; base < cnt; base += bd * U) {
+    u64 k[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = base + u * bd + threadIdx.x;
+      k[u] = i < cnt ? src[i] : ~0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) f(k[u], base + u * bd + threadIdx.x < cnt);
+  }
+}
+
+// LDS histogram increment with one atomic per wave when all active lanes hit
+// the same bin (the common case: runs of equal scores).
+__device__ __forceinline__ void lds_count(uint32_t* hist, uint32_t b, bool in, int lane) {
+  const u64 act = __ballot(in);
+  if (!act) return;
+  const int l0 = __ffsll((long long)act) - 1;
+  const uint32_t b0 = (uint32_t)__shfl((int)b, l0);
+  if (__ballot(in && b == b0) == act) {
+    if (lane == l0) atomicAdd(&hist[b0], (uint32_t)__popcll(act));
+  } else if (in) {
+    atomicAdd(&hist[b], 1u);
+  }
+}
+
+// Wave-aggregated append: returns this lane's slot (valid where `take`).
+__device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool take, int lane) {
+  const u64 m = __ballot(take);
+  if (!m) return 0;
+  const int l0 = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == l0) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+  base = (uint32_t)__shfl((int)base, l0);
+  return base + lanes_below(m, lane);
+}
+
+// First selection digit.  Monotone non-decreasing in the key whatever lo and
+// s1 are, so the selection is exact for any parameters; lo and s1 only decide
+// how finely the keys near the threshold are spread over the bins.
+__device__ __forceinline__ uint32_t digit1(uint32_t hi, uint32_t lo, uint32_t s1) {
+  if (hi < lo) return 0;
+  const uint32_t d = (hi - lo) >> s1;
+  return d < (uint32_t)kNB ? d : (uint32_t)(kNB - 1);
 }
 
 // What changes from a parent snapshot to one child.
@@ -114,9 +191,13 @@ __global__ void k_init(LevelArgs a) {
     root[i] = v;
   }
   for (int i = threadIdx.x; i < kNB; i += blockDim.x) {
-    a.hist[i] = 0; a.fill[i] = 0; a.hist_next[i] = 0; a.fill_next[i] = 0;
+    a.hist[i] = 0;
+    a.hist_next[i] = 0;
   }
   if (threadIdx.x == 0) {
+    SelState z = {};
+    *a.sel = z;
+    *a.sel_next = z;
     a.pcount[0] = 1;
     *a.expanded = 0ull;
     a.result[0] = a.best_known; a.result[1] = 0; a.result[2] = 0;
@@ -129,8 +210,11 @@ __global__ void k_init(LevelArgs a) {
 // MODE 2 = PMS_GPU (ops on the hash-chosen pivot machine, gpu/v1/solver.cu:47-70).
 template <int MODE, bool SINGLE>
 __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
-  __shared__ uint32_t s_red[kScoreWaves][4];
+  __shared__ uint32_t s_h[kNB];
+  __shared__ uint32_t s_cnt[kScoreWaves], s_exp[kScoreWaves];
   __shared__ u64 s_best[kScoreWaves];
+  __shared__ int s_rmin[kScoreWaves];
+  __shared__ uint32_t s_rmax[kScoreWaves], s_rlo[kScoreWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const Layout Ly = make_layout(a.n, a.m);
   const int n = a.n, m = a.m, S = Ly.S, B = a.B;
@@ -141,7 +225,49 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
   const int NW = (S + 63) >> 6;
   constexpr int ORDER = (MODE == 1) ? kOrderErase : kOrderSwap;
 
-  uint32_t w_cnt = 0, w_min = 0xFFFFFFFFu, w_max = 0, w_exp = 0;
+  // Digit parameters.  Every child's score is >= fl(cmin / ns), cmin = the
+  // smallest parent cmax: cmax only grows, correctly rounded division is
+  // monotone, and adding the non-negative noise cannot round below the base.
+  // The shift spreads the previous level's survivors' score range over
+  // ~2^kSpreadBins bins.
+  uint32_t lo = 0, s1 = kLevel0Shift;
+  if (a.level > 0 && !last) {
+    int cmin = INT_MAX;
+    uint32_t mxh = 0, mnh = 0xFFFFFFFFu;
+    for (int i = threadIdx.x; i < a.nblk_rank; i += blockDim.x) {
+      cmin = min(cmin, a.rstat[i].min_cmax);
+      mxh = max(mxh, a.rstat[i].max_hi);
+      mnh = min(mnh, a.rstat[i].min_hi);
+    }
+    cmin = wave_min_i(cmin);
+    mxh = wave_max(mxh);
+    mnh = wave_min_u(mnh);
+    if (lane == 0) { s_rmin[wave] = cmin; s_rmax[wave] = mxh; s_rlo[wave] = mnh; }
+    __syncthreads();
+    cmin = INT_MAX;
+    mxh = 0;
+    mnh = 0xFFFFFFFFu;
+    for (int w = 0; w < kScoreWaves; ++w) {
+      cmin = min(cmin, s_rmin[w]);
+      mxh = max(mxh, s_rmax[w]);
+      mnh = min(mnh, s_rlo[w]);
+    }
+    if (a.exact_digit) {
+      lo = cmin == INT_MAX ? 0u : (uint32_t)cmin;
+    } else {
+      if (cmin != INT_MAX) lo = __float_as_uint((float)cmin / (float)ns);
+      const uint32_t spread = mxh > mnh ? mxh - mnh : 0u;
+      const int bl = spread ? 32 - __clz(spread) : 0;
+      s1 = (uint32_t)max(0, bl - kSpreadBins);
+    }
+  } else if (a.exact_digit) {
+    lo = 0;  // the root's cmax
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) { a.sel->lo = lo; a.sel->s1 = s1; a.sel->exact = a.exact_digit ? 1u : 0u; }
+  for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
+
+  uint32_t w_cnt = 0, w_exp = 0;
   u64 w_best = ~0ull;
 
   for (int p = blockIdx.x * kScoreWaves + wave; p < P; p += gridDim.x * kScoreWaves) {
@@ -216,24 +342,51 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       keep[h] = val[h] && cm[h] < a.best_known;
-      if (!val[h]) continue;
-      const uint32_t x = (uint32_t)(p * B + slot[h]);
-      const int j = h * 64 + lane;
-      if (keep[h]) {
-        a.cjob[x] = (uint8_t)j;
-        if (last) {
-          const u64 c = ((u64)(uint32_t)cm[h] << 32) | x;
-          w_best = c < w_best ? c : w_best;
-        } else {
-          const float sc = gpu_score(cm[h], ns, noise_hash(a.lseed, (uint32_t)p, (uint32_t)slot[h]));
-          const uint32_t bits = __float_as_uint(sc);
-          a.keys[x] = ((u64)bits << 32) | x;
-          w_cnt++;
-          w_min = min(w_min, bits);
-          w_max = max(w_max, bits);
+      uint32_t d = 0;
+      bool counted = false;
+      if (val[h]) {
+        const uint32_t x = (uint32_t)(p * B + slot[h]);
+        const int j = h * 64 + lane;
+        if (keep[h]) {
+          a.cjob[x] = (uint8_t)j;
+          if (last) {
+            const u64 c = ((u64)(uint32_t)cm[h] << 32) | x;
+            w_best = c < w_best ? c : w_best;
+          } else {
+            const float base = gpu_score_base(cm[h], ns);
+            const float sc = gpu_score_add_noise(base, noise_hash(a.lseed, (uint32_t)p, (uint32_t)slot[h]));
+            const uint32_t bits = __float_as_uint(sc);
+            a.keys[x] = ((u64)bits << 32) | x;
+            if (a.exact_digit) {
+              // Monotone in (score, child) when a lower cmax always scores
+              // lower (checked per instance on the host): cmax >= lo = cmin,
+              // and for one cmax the score grows with its ulp offset.
+              const uint32_t dc = min((uint32_t)(cm[h] - (int)lo), 256u);
+              const uint32_t off = min(bits - __float_as_uint(base), 7u);
+              d = min(8u * dc + off, (uint32_t)(kNB - 1));
+            } else {
+              d = digit1(bits, lo, s1);
+            }
+            a.kdig[x] = (uint16_t)d;
+            counted = true;
+            w_cnt++;
+          }
+        } else if (!last) {
+          a.keys[x] = ~0ull;
         }
-      } else if (!last) {
-        a.keys[x] = ~0ull;
+      }
+      // LDS histogram; one atomic for the whole wave when its digits agree
+      // (the usual case: most children keep their parent's cmax).
+      const u64 cm_mask = __ballot(counted);
+      if (cm_mask) {
+        const int l0 = __ffsll((long long)cm_mask) - 1;
+        const uint32_t d0 = (uint32_t)__shfl((int)d, l0);
+        const u64 same = __ballot(counted && d == d0);
+        if (same == cm_mask) {
+          if (lane == l0) atomicAdd(&s_h[d0], (uint32_t)__popcll(cm_mask));
+        } else if (counted) {
+          atomicAdd(&s_h[d], 1u);
+        }
       }
     }
     if (!last) {
@@ -283,218 +436,69 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
     }
   }
 
-  // Block partials: kept count, score-bit range, expanded count, last-level best.
+  // Block partials: kept count, expanded count, last-level best; histogram.
   w_cnt = wave_sum(w_cnt);
-  w_min = wave_min(w_min);
-  w_max = wave_max(w_max);
   w_best = wave_min64(w_best);
-  if (lane == 0) {
-    s_red[wave][0] = w_cnt; s_red[wave][1] = w_min; s_red[wave][2] = w_max; s_red[wave][3] = w_exp;
-    s_best[wave] = w_best;
-  }
+  if (lane == 0) { s_cnt[wave] = w_cnt; s_exp[wave] = w_exp; s_best[wave] = w_best; }
   __syncthreads();
   if (threadIdx.x == 0) {
     BlockStat b;
-    b.valid = 0; b.minhi = 0xFFFFFFFFu; b.maxhi = 0; b.expanded = 0; b.best = ~0ull;
+    b.valid = 0; b.expanded = 0; b.best = ~0ull;
     for (int w = 0; w < kScoreWaves; ++w) {
-      b.valid += s_red[w][0];
-      b.minhi = min(b.minhi, s_red[w][1]);
-      b.maxhi = max(b.maxhi, s_red[w][2]);
-      b.expanded += s_red[w][3];
+      b.valid += s_cnt[w];
+      b.expanded += s_exp[w];
       b.best = s_best[w] < b.best ? s_best[w] : b.best;
     }
     a.bstat[blockIdx.x] = b;
   }
-}
-
-// Reduces the score blocks' partials (every caller block does it redundantly,
-// which avoids a separate launch and any same-address atomics).
-struct Totals {
-  uint32_t valid, minhi, maxhi, expanded;
-  u64 best;
-};
-
-__device__ Totals reduce_bstat(const BlockStat* bs, int nb) {
-  __shared__ uint32_t s_v[16], s_mn[16], s_mx[16], s_ex[16];
-  __shared__ u64 s_b[16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t v = 0, mn = 0xFFFFFFFFu, mx = 0, ex = 0;
-  u64 b = ~0ull;
-  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
-    const BlockStat s = bs[i];
-    v += s.valid; mn = min(mn, s.minhi); mx = max(mx, s.maxhi); ex += s.expanded;
-    b = s.best < b ? s.best : b;
-  }
-  v = wave_sum(v); mn = wave_min(mn); mx = wave_max(mx); ex = wave_sum(ex); b = wave_min64(b);
-  if (lane == 0) { s_v[wave] = v; s_mn[wave] = mn; s_mx[wave] = mx; s_ex[wave] = ex; s_b[wave] = b; }
-  __syncthreads();
-  Totals t;
-  t.valid = 0; t.minhi = 0xFFFFFFFFu; t.maxhi = 0; t.expanded = 0; t.best = ~0ull;
-  const int nw = (blockDim.x + 63) >> 6;
-  for (int w = 0; w < nw; ++w) {
-    t.valid += s_v[w]; t.minhi = min(t.minhi, s_mn[w]); t.maxhi = max(t.maxhi, s_mx[w]);
-    t.expanded += s_ex[w]; t.best = s_b[w] < t.best ? s_b[w] : t.best;
-  }
-  __syncthreads();
-  return t;
+  if (!last)
+    for (int i = threadIdx.x; i < kNB; i += blockDim.x)
+      if (s_h[i]) atomicAdd(&a.hist[i], s_h[i]);
 }
 
 // ---------------------------------------------------------------------------
-// Histogram of the first digit over all kept keys of the level.
-__global__ __launch_bounds__(1024) void k_hist(LevelArgs a) {
-  __shared__ uint32_t s_h[kNB];
-  const Totals t = reduce_bstat(a.bstat, a.nblk_score);
-  const uint32_t range = t.maxhi - t.minhi;
-  const int bitlen = (t.valid == 0 || range == 0) ? 0 : 32 - __clz(range);
-  const uint32_t shift = (uint32_t)max(0, bitlen - kLog2NB);
-  const uint32_t Kp = min(t.valid, (uint32_t)a.K);
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    SelState s;
-    s.V = t.valid; s.lo = t.minhi; s.hi = t.maxhi; s.shift = shift; s.Kp = Kp;
-    s.T = 0; s.cumlt = 0; s.need = 0; s.t = 0;
-    *a.sel = s;
-    a.pcount[a.level + 1] = (int)Kp;
-    *a.expanded += (u64)t.expanded;
-    a.lvl_exp[a.level] = t.expanded;
-  }
-  if (t.valid == 0) return;
-  for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
-  __syncthreads();
-  const size_t N = (size_t)a.pcount[a.level] * a.B;
-  for (size_t x = (size_t)blockIdx.x * blockDim.x + threadIdx.x; x < N; x += (size_t)gridDim.x * blockDim.x) {
-    const u64 key = a.keys[x];
-    if (key != ~0ull) atomicAdd(&s_h[sel_digit(key, t.minhi, shift)], 1u);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < kNB; i += blockDim.x)
-    if (s_h[i]) atomicAdd(&a.hist[i], s_h[i]);
-}
-
-// ---------------------------------------------------------------------------
-// Exclusive scan of the histogram (redundantly per block), threshold bin T,
-// and the scatter of every key at or below T into its bin's bucket.
-__global__ __launch_bounds__(1024) void k_partition(LevelArgs a) {
-  __shared__ uint32_t s_ex[kNB];
-  __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_T;
-  const SelState sel = *a.sel;
-  if (sel.Kp == 0) return;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) s_T = kNB - 1;
-  // 1024 threads x 2 bins
-  const uint32_t h0 = a.hist[2 * tid], h1 = a.hist[2 * tid + 1];
-  const uint32_t sum = h0 + h1;
-  const uint32_t inc = wave_incl_scan(sum, lane);
-  if (lane == 63) s_w[wave] = inc;
-  __syncthreads();
-  uint32_t wofs = 0;
-  for (int w = 0; w < wave; ++w) wofs += s_w[w];
-  const uint32_t ex0 = wofs + inc - sum;
-  s_ex[2 * tid] = ex0;
-  s_ex[2 * tid + 1] = ex0 + h0;
-  if (ex0 < sel.Kp && ex0 + h0 >= sel.Kp) s_T = 2 * tid;
-  if (ex0 + h0 < sel.Kp && ex0 + h0 + h1 >= sel.Kp) s_T = 2 * tid + 1;
-  __syncthreads();
-  const uint32_t T = s_T;
-  if (blockIdx.x == 0) {
-    a.excl[2 * tid] = s_ex[2 * tid];
-    a.excl[2 * tid + 1] = s_ex[2 * tid + 1];
-    if (tid == 0) {
-      SelState* g = a.sel;
-      g->T = T;
-      g->cumlt = s_ex[T];
-      g->need = sel.Kp - s_ex[T];
-      g->t = a.hist[T];
-    }
-  }
-  const size_t N = (size_t)a.pcount[a.level] * a.B;
-  for (size_t x = (size_t)blockIdx.x * blockDim.x + tid; x < N; x += (size_t)gridDim.x * blockDim.x) {
-    const u64 key = a.keys[x];
-    if (key == ~0ull) continue;
-    const uint32_t d = sel_digit(key, sel.lo, sel.shift);
-    if (d <= T) {
-      const uint32_t pos = s_ex[d] + atomicAdd(&a.fill[d], 1u);
-      a.bucketed[pos] = key;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Bitonic sort of n (power of two) keys in LDS by one block.
-__device__ void lds_bitonic(u64* s, int n) {
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const u64 x = s[i], y = s[ixj];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) { s[i] = y; s[ixj] = x; }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// Same in global memory (one block; only for degenerate threshold bins).
-__device__ void global_bitonic(u64* s, int n) {
-  for (int k = 2; k <= n; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        const int ixj = i ^ j;
-        if (ixj > i) {
-          const u64 x = s[i], y = s[ixj];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) { s[i] = y; s[ixj] = x; }
-        }
-      }
-      __threadfence_block();
-      __syncthreads();
-    }
-  }
-}
-
-__device__ __forceinline__ int next_pow2(int v) {
-  int p = 1;
-  while (p < v) p <<= 1;
-  return p;
-}
-
-// Orders the threshold bin: its `need` smallest keys, sorted, are written to
-// the front of its bucket.  One block.  Bins of up to kTieLds keys are sorted
-// in LDS; larger ones (degenerate inputs) are first cut to `need` keys by an
-// 8-bit MSD radix select over the full 64-bit key.
-__global__ __launch_bounds__(1024) void k_tie(LevelArgs a) {
-  __shared__ u64 s_k[kTieLds];
+// 8-bit MSD radix select over the full 64-bit key, by one block: keeps the
+// `need` smallest of the t keys of R in R[0, need) (order unspecified).
+__device__ void refine_threshold(u64* R, uint32_t t, uint32_t need, u64* scratch) {
+  __shared__ u64 s_mn[16], s_mx[16];
   __shared__ uint32_t s_h[256];
   __shared__ u64 s_want, s_mask;
   __shared__ uint32_t s_kth, s_cnt;
-  const SelState sel = *a.sel;
-  if (sel.Kp == 0) return;
-  u64* region = a.bucketed + sel.cumlt;
-  const int t = (int)sel.t, need = (int)sel.need;
-  if (t <= kTieLds) {
-    const int np = next_pow2(t);
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s_k[i] = (i < t) ? region[i] : ~0ull;
-    __syncthreads();
-    lds_bitonic(s_k, np);
-    for (int i = threadIdx.x; i < need; i += blockDim.x) region[i] = s_k[i];
-    return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = (int)(blockDim.x >> 6);
+  u64 mn = ~0ull, mx = 0;
+  for (uint32_t i = tid; i < t; i += blockDim.x) {
+    const u64 k = R[i];
+    mn = k < mn ? k : mn;
+    mx = k > mx ? k : mx;
   }
-  // Radix select of the need-th smallest key.
-  if (threadIdx.x == 0) { s_want = 0; s_mask = 0; s_kth = (uint32_t)(need - 1); }
+  mn = wave_min64(mn);
+  mx = wave_max64(mx);
+  if (lane == 0) { s_mn[wave] = mn; s_mx[wave] = mx; }
   __syncthreads();
-  for (int shift = 56; shift >= 0; shift -= 8) {
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_h[i] = 0;
+  if (tid == 0) {
+    u64 a = ~0ull, b = 0;
+    for (int w = 0; w < nw; ++w) { a = s_mn[w] < a ? s_mn[w] : a; b = s_mx[w] > b ? s_mx[w] : b; }
+    const int hb = 63 - __clzll((long long)(a ^ b));  // keys are distinct and t >= 2
+    const int start = hb & ~7;
+    const u64 mask = (start + 8 >= 64) ? 0ull : (~0ull << (start + 8));
+    s_mask = mask;
+    s_want = a & mask;
+    s_kth = need - 1;
+    s_cnt = (uint32_t)start;  // first digit position (reused below as a counter)
+  }
+  __syncthreads();
+  const int start = (int)s_cnt;
+  for (int shift = start; shift >= 0; shift -= 8) {
+    for (int i = tid; i < 256; i += blockDim.x) s_h[i] = 0;
     __syncthreads();
     const u64 want = s_want, mask = s_mask;
-    for (int i = threadIdx.x; i < t; i += blockDim.x) {
-      const u64 k = region[i];
+    for (uint32_t i = tid; i < t; i += blockDim.x) {
+      const u64 k = R[i];
       if ((k & mask) == want) atomicAdd(&s_h[(k >> shift) & 255], 1u);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       uint32_t kth = s_kth, cum = 0;
       int d = 0;
       while (d < 255 && cum + s_h[d] <= kth) { cum += s_h[d]; ++d; }
@@ -505,36 +509,303 @@ __global__ __launch_bounds__(1024) void k_tie(LevelArgs a) {
     __syncthreads();
   }
   const u64 thr = s_want;
-  if (threadIdx.x == 0) s_cnt = 0;
+  if (tid == 0) s_cnt = 0;
   __syncthreads();
-  u64* scratch = a.keys;
-  for (int i = threadIdx.x; i < t; i += blockDim.x) {
-    const u64 k = region[i];
-    if (k <= thr) scratch[atomicAdd(&s_cnt, 1u)] = k;
+  for (uint32_t i = tid; i < t; i += blockDim.x) {
+    const u64 k = R[i];
+    if (k <= thr) {
+      const uint32_t at = atomicAdd(&s_cnt, 1u);
+      if (at < need) scratch[at] = k;
+    }
   }
   __threadfence_block();
   __syncthreads();
-  const int np = next_pow2(need);
-  if (np <= kTieLds) {
-    for (int i = threadIdx.x; i < np; i += blockDim.x) s_k[i] = (i < need) ? scratch[i] : ~0ull;
+  for (uint32_t i = tid; i < need; i += blockDim.x) R[i] = scratch[i];
+}
+
+// Exact cut of the threshold bin, by one block.  Each round histograms the
+// score bits of the remaining candidates into kSubBins bins spanning exactly
+// their value range: the keys of the bins below the one holding the cut
+// survive, the keys of that bin are the next round's candidates (the range
+// shrinks 4096x per round, so after at most three rounds a bin holds a single
+// score value).  Among the keys of the threshold score, the survivors are the
+// smallest child indices: the need'-th set bit of an LDS bitmap over the child
+// index.  Leaves the `need` smallest of the t keys of R in R[0, need).
+constexpr int kSubBins = 4096;
+constexpr int kLog2SubBins = 12;
+__device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bufB, u64* outb,
+                           uint32_t* bitmap, uint32_t bm_words, uint32_t N) {
+  __shared__ uint32_t s_sub[kSubBins];
+  __shared__ uint32_t s_mn[16], s_mx[16], s_w[16];
+  __shared__ uint32_t s_b, s_needb, s_cntb, s_I, s_out, s_nxt;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nw = (int)(blockDim.x >> 6);
+  const int per = kSubBins / (int)blockDim.x;
+  if (tid == 0) s_out = 0;
+  const u64* src = R;
+  uint32_t cnt = t, nd = need;
+  u64* nxt = bufA;
+  for (int round = 0; round < 4; ++round) {
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    for_keys(src, cnt, [&](u64 k, bool in) {
+      if (in) {
+        const uint32_t h = (uint32_t)(k >> 32);
+        mn = min(mn, h);
+        mx = max(mx, h);
+      }
+    });
+    mn = wave_min_u(mn);
+    mx = wave_max(mx);
+    if (lane == 0) { s_mn[wave] = mn; s_mx[wave] = mx; }
+    for (int i = tid; i < kSubBins; i += blockDim.x) s_sub[i] = 0;
     __syncthreads();
-    lds_bitonic(s_k, np);
-    for (int i = threadIdx.x; i < need; i += blockDim.x) region[i] = s_k[i];
-  } else {
-    for (int i = need + threadIdx.x; i < np; i += blockDim.x) scratch[i] = ~0ull;
+    mn = 0xFFFFFFFFu; mx = 0;
+    for (int w = 0; w < nw; ++w) { mn = min(mn, s_mn[w]); mx = max(mx, s_mx[w]); }
+    const uint32_t range = mx - mn;
+    const int bl = range ? 32 - __clz(range) : 0;
+    const uint32_t sh = (uint32_t)max(0, bl - kLog2SubBins);
+    for_keys(src, cnt, [&](u64 k, bool in) {
+      const uint32_t b = in ? (((uint32_t)(k >> 32) - mn) >> sh) : 0u;
+      lds_count(s_sub, b, in, lane);
+    });
+    __syncthreads();
+    uint32_t loc = 0;
+    for (int k = 0; k < per; ++k) loc += s_sub[tid * per + k];
+    const uint32_t inc = wave_incl_scan(loc, lane);
+    if (lane == 63) s_w[wave] = inc;
+    __syncthreads();
+    uint32_t ex = inc - loc;
+    for (int w = 0; w < wave; ++w) ex += s_w[w];
+    for (int k = 0; k < per; ++k) {
+      const uint32_t c = s_sub[tid * per + k];
+      if (ex < nd && ex + c >= nd) { s_b = (uint32_t)(tid * per + k); s_needb = nd - ex; s_cntb = c; }
+      ex += c;
+    }
+    __syncthreads();
+    const uint32_t bstar = s_b, needb = s_needb, cntb = s_cntb;
+    if (sh == 0 || needb == cntb) {
+      // Final round: bins below bstar survive, and from bin bstar (one score
+      // value when sh == 0) either all keys or the needb smallest indices.
+      uint32_t I = 0xFFFFFFFFu;
+      if (needb < cntb) {
+        const uint32_t W = (N + 31) / 32;
+        if (W > bm_words) {
+          // No room for the bitmap: radix-select the nd smallest of src.
+          u64* tmp = (src == bufA) ? bufB : bufA;
+          refine_threshold(const_cast<u64*>(src), cnt, nd, tmp);
+          const uint32_t o = s_out;
+          for (uint32_t i = tid; i < nd; i += blockDim.x) outb[o + i] = src[i];
+          __threadfence_block();
+          __syncthreads();
+          break;
+        }
+        const uint32_t V = mn + bstar;
+        for (uint32_t i = tid; i < W; i += blockDim.x) bitmap[i] = 0;
+        __syncthreads();
+        for_keys(src, cnt, [&](u64 k, bool in) {
+          if (in && (uint32_t)(k >> 32) == V) {
+            const uint32_t x = (uint32_t)k;
+            atomicOr(&bitmap[x >> 5], 1u << (x & 31));
+          }
+        });
+        __syncthreads();
+        const uint32_t wpt = (W + blockDim.x - 1) / blockDim.x;
+        const uint32_t w0 = min(W, tid * wpt), w1 = min(W, w0 + wpt);
+        uint32_t pc = 0;
+        for (uint32_t w = w0; w < w1; ++w) pc += (uint32_t)__popc(bitmap[w]);
+        const uint32_t pinc = wave_incl_scan(pc, lane);
+        if (lane == 63) s_w[wave] = pinc;
+        __syncthreads();
+        uint32_t pex = pinc - pc;
+        for (int w = 0; w < wave; ++w) pex += s_w[w];
+        if (pex < needb && pex + pc >= needb) {
+          uint32_t left = needb - pex;  // 1-based rank inside this thread's words
+          for (uint32_t w = w0; w < w1; ++w) {
+            uint32_t bits = bitmap[w];
+            const uint32_t c = (uint32_t)__popc(bits);
+            if (left > c) { left -= c; continue; }
+            for (uint32_t q = 1; q < left; ++q) bits &= bits - 1;
+            s_I = w * 32 + (uint32_t)(__ffs((int)bits) - 1);
+            break;
+          }
+        }
+        __syncthreads();
+        I = s_I;
+      }
+      for_keys(src, cnt, [&](u64 k, bool in) {
+        const uint32_t b = in ? ((uint32_t)(k >> 32) - mn) >> sh : 0u;
+        const bool keep = in && (b < bstar || (b == bstar && (uint32_t)k <= I));
+        const uint32_t at = wave_append(&s_out, keep, lane);
+        if (keep) outb[at] = k;
+      });
+      __threadfence_block();
+      __syncthreads();
+      break;
+    }
+    // Narrow: keys of lower bins survive, keys of bin bstar go on.
+    if (tid == 0) s_nxt = 0;
+    __syncthreads();
+    for_keys(src, cnt, [&](u64 k, bool in) {
+      const uint32_t b = in ? ((uint32_t)(k >> 32) - mn) >> sh : 0u;
+      const bool lower = in && b < bstar, same = in && b == bstar;
+      const uint32_t at = wave_append(&s_out, lower, lane);
+      if (lower) outb[at] = k;
+      const uint32_t at2 = wave_append(&s_nxt, same, lane);
+      if (same) nxt[at2] = k;
+    });
     __threadfence_block();
     __syncthreads();
-    global_bitonic(scratch, np);
-    for (int i = threadIdx.x; i < need; i += blockDim.x) region[i] = scratch[i];
+    src = nxt;
+    cnt = cntb;
+    nd = needb;
+    nxt = (nxt == bufA) ? bufB : bufA;
+  }
+  for (uint32_t i = tid; i < need; i += blockDim.x) R[i] = outb[i];
+}
+
+// Threshold bin from the histogram, block-aggregated compaction of the keys
+// below it (sure survivors) and in it (ties); the last block cuts the ties to
+// exactly the survivors, so cand[0, Kp) holds the Kp survivors.
+__global__ __launch_bounds__(kFilterThreads) void k_filter(LevelArgs a) {
+  __shared__ uint32_t s_ex[kNB];
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_v[16], s_e[16];
+  __shared__ uint32_t s_T;
+  __shared__ uint32_t s_off[2][16];
+  __shared__ uint32_t s_base[2];
+  __shared__ int s_last;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  SelState* sel = a.sel;
+
+  uint32_t v = 0, ex = 0;
+  for (int i = tid; i < a.nblk_score; i += blockDim.x) { v += a.bstat[i].valid; ex += a.bstat[i].expanded; }
+  v = wave_sum(v);
+  ex = wave_sum(ex);
+  if (lane == 0) { s_v[wave] = v; s_e[wave] = ex; }
+  if (tid == 0) s_T = kNB - 1;
+  __syncthreads();
+  v = 0; ex = 0;
+  for (int w = 0; w < 16; ++w) { v += s_v[w]; ex += s_e[w]; }
+  const uint32_t Kp = min(v, (uint32_t)a.K);
+  if (blockIdx.x == 0 && tid == 0) {
+    sel->V = v;
+    sel->Kp = Kp;
+    a.pcount[a.level + 1] = (int)Kp;
+    *a.expanded += (u64)ex;
+    a.lvl_exp[a.level] = ex;
+  }
+  if (Kp == 0) return;
+  const uint32_t s1 = sel->s1;
+  const bool exact = sel->exact != 0;
+
+  // Exclusive scan of the histogram: 1024 threads x 2 bins.
+  const uint32_t h0 = a.hist[2 * tid], h1 = a.hist[2 * tid + 1];
+  const uint32_t sum = h0 + h1;
+  const uint32_t inc = wave_incl_scan(sum, lane);
+  if (lane == 63) s_w[wave] = inc;
+  __syncthreads();
+  uint32_t wofs = 0;
+  for (int w = 0; w < wave; ++w) wofs += s_w[w];
+  const uint32_t ex0 = wofs + inc - sum;
+  s_ex[2 * tid] = ex0;
+  s_ex[2 * tid + 1] = ex0 + h0;
+  if (ex0 < Kp && ex0 + h0 >= Kp) s_T = 2 * tid;
+  if (ex0 + h0 < Kp && ex0 + h0 + h1 >= Kp) s_T = 2 * tid + 1;
+  __syncthreads();
+  const uint32_t T1 = s_T;
+  const uint32_t cum_lt = s_ex[T1];
+  const uint32_t t1 = a.hist[T1];
+  const uint32_t need = Kp - cum_lt;
+  // A single score value in the threshold bin: its survivors are the `need`
+  // smallest child indices, so ties are only marked in a bitmap (by ballot,
+  // 64 children per wave = two words) and cut by prefix popcount.
+  const bool single = exact ? ((T1 & 7u) != 7u) : (s1 == 0 && T1 < (uint32_t)(kNB - 1));
+  if (blockIdx.x == 0 && tid == 0) {
+    sel->T1 = T1; sel->cum_lt = cum_lt; sel->t1 = t1; sel->need = need; sel->single = single ? 1u : 0u;
+  }
+
+  const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
+  const uint32_t W = (N + 31) / 32;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
+    const uint32_t x = base + tid;
+    const u64 key = x < N ? a.keys[x] : ~0ull;
+    uint32_t d = kNB;
+    if (key != ~0ull) d = a.kdig[x];
+    const bool sure = d < T1;
+    const bool tieb = d == T1;
+    if (single) {
+      const u64 tb = __ballot(tieb);
+      const uint32_t w0 = (base + 64u * (uint32_t)wave) >> 5;
+      if (lane == 0 && w0 < W) {
+        a.tbits[w0] = (uint32_t)tb;
+        if (w0 + 1 < W) a.tbits[w0 + 1] = (uint32_t)(tb >> 32);
+      }
+    }
+    const bool tie = tieb && !single;
+    const u64 bs = __ballot(sure), bt = __ballot(tie);
+    if (lane == 0) { s_off[0][wave] = (uint32_t)__popcll(bs); s_off[1][wave] = (uint32_t)__popcll(bt); }
+    __syncthreads();
+    if (tid == 0) {
+      uint32_t ts = 0, tt = 0;
+      for (int w = 0; w < 16; ++w) {
+        const uint32_t cs = s_off[0][w], ct = s_off[1][w];
+        s_off[0][w] = ts; s_off[1][w] = tt;
+        ts += cs; tt += ct;
+      }
+      s_base[0] = ts ? atomicAdd(&sel->sure_ctr, ts) : 0u;
+      s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
+    }
+    __syncthreads();
+    if (sure) a.cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
+    if (tie) a.cand[s_base[1] + s_off[1][wave] + lanes_below(bt, lane)] = key;
+    __syncthreads();
+  }
+
+  // The last block to finish sees every block's candidates.
+  if (tid == 0) {
+    __threadfence();
+    const uint32_t ticket = atomicAdd(&sel->done_ctr, 1u);
+    s_last = (ticket == gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  if (single) {
+    // Ties in child-index order: the first `need` set bits of the bitmap.
+    const uint32_t wpt = (W + blockDim.x - 1) / blockDim.x;
+    const uint32_t w0 = min(W, tid * wpt), w1 = min(W, w0 + wpt);
+    uint32_t pc = 0;
+    for (uint32_t w = w0; w < w1; ++w) pc += (uint32_t)__popc(a.tbits[w]);
+    const uint32_t pinc = wave_incl_scan(pc, lane);
+    if (lane == 63) s_w[wave] = pinc;
+    __syncthreads();
+    uint32_t r = pinc - pc;
+    for (int w = 0; w < wave; ++w) r += s_w[w];
+    for (uint32_t w = w0; w < w1 && r < need; ++w) {
+      uint32_t bits = a.tbits[w];
+      while (bits && r < need) {
+        const uint32_t x = w * 32 + (uint32_t)(__ffs((int)bits) - 1);
+        bits &= bits - 1;
+        a.cand[cum_lt + r] = a.keys[x];
+        ++r;
+      }
+    }
+    if (tid == 0) sel->sorted_from = cum_lt;
+  } else {
+    if (need < t1) {
+      extern __shared__ uint32_t s_bitmap[];
+      refine_cut(a.cand + cum_lt, t1, need, a.keys, a.ref_buf, a.ref_buf + N, s_bitmap, a.bm_words, N);
+    }
+    if (tid == 0) sel->sorted_from = Kp;
   }
 }
 
 // ---------------------------------------------------------------------------
 // Builds survivor q of the next beam from its parent (two-phase) and writes
-// the trail entry.  One wave.
+// the trail entry.  One wave.  Returns the child's cmax.
 template <int ORDER>
-__device__ __forceinline__ void materialize_one(const LevelArgs& a, const Layout& Ly, int lane,
-                                                int p, int j, int q) {
+__device__ __forceinline__ int materialize_one(const LevelArgs& a, const Layout& Ly, int lane, int p, int j,
+                                               int q) {
   const int S = Ly.S, m = a.m;
   const int NW = (S + 63) >> 6;
   const int* par = a.beam_cur + (size_t)p * S;
@@ -576,47 +847,81 @@ __device__ __forceinline__ void materialize_one(const LevelArgs& a, const Layout
     if (i < NW && o < S) out[o] = child_word<ORDER>(o, pw[i], c, Ly);
   }
   if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | j;
+  return c.cm;
 }
 
-// Rank of every survivor inside its bucket (count of smaller keys), then
+// Rank of every survivor among the Kp survivors (count of smaller keys), then
 // materialise (two-phase) or copy from the child buffer (single-phase) into
 // slot q of the next beam.  One wave per survivor.
 template <int ORDER, bool SINGLE>
-__global__ __launch_bounds__(256) void k_rank(LevelArgs a) {
+__global__ __launch_bounds__(64 * kRankWaves) void k_rank(LevelArgs a) {
+  extern __shared__ u64 s_c[];
+  __shared__ int s_mc[kRankWaves];
+  __shared__ uint32_t s_mh[kRankWaves], s_ml[kRankWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNB; i += gridDim.x * blockDim.x) {
-    a.hist_next[i] = 0;
-    a.fill_next[i] = 0;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNB; i += gridDim.x * blockDim.x) a.hist_next[i] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    a.sel_next->sure_ctr = 0;
+    a.sel_next->tie_ctr = 0;
+    a.sel_next->done_ctr = 0;
+    if (a.sel_log) a.sel_log[a.level] = *a.sel;
   }
-  const SelState sel = *a.sel;
-  const int Kp = (int)sel.Kp;
+  const int Kp = (int)a.sel->Kp;
+  // cand[0, sf) holds keys below every key of cand[sf, Kp), which is already
+  // in order: only the first part needs ranking.
+  const int sf = min((int)a.sel->sorted_from, Kp);
+  const bool in_lds = a.K <= kRankLdsMax;
+  if (in_lds) {
+    for (int i = threadIdx.x; i < sf; i += blockDim.x) s_c[i] = a.cand[i];
+    __syncthreads();
+  }
+  const u64* C = in_lds ? s_c : a.cand;
   const Layout Ly = make_layout(a.n, a.m);
   const int S = Ly.S;
+  int w_min = INT_MAX;
+  uint32_t w_max = 0, w_minh = 0xFFFFFFFFu;
   for (int u = blockIdx.x * kRankWaves + wave; u < Kp; u += gridDim.x * kRankWaves) {
-    const u64 key = a.bucketed[u];
-    int q = u;
-    if ((uint32_t)u < sel.cumlt) {
-      const uint32_t d = sel_digit(key, sel.lo, sel.shift);
-      const uint32_t bs = a.excl[d], b = a.hist[d];
-      uint32_t cnt = 0;
-      for (uint32_t base = 0; base < b; base += 64) {
-        const uint32_t i = base + lane;
-        const bool lt = (i < b) && a.bucketed[bs + i] < key;
+    const u64 key = u < sf ? C[u] : a.cand[u];
+    uint32_t cnt = (uint32_t)u;
+    if (u < sf) {
+      cnt = 0;
+      for (int base = 0; base < sf; base += 64) {
+        const int v = base + lane;
+        const bool lt = v < sf && C[v] < key;
         cnt += (uint32_t)__popcll(__ballot(lt));
       }
-      q = (int)(bs + cnt);
     }
+    const int q = (int)cnt;
     const uint32_t x = (uint32_t)key;
     const int p = (int)(x / (uint32_t)a.B);
     const int j = a.cjob[x];
+    int cm;
     if (SINGLE) {
       const int* src = a.cbuf + (size_t)x * S;
       int* dst = a.beam_next + (size_t)q * S;
       for (int o = lane; o < S; o += 64) dst[o] = src[o];
+      cm = src[Ly.off_cmax];
       if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | j;
     } else {
-      materialize_one<ORDER>(a, Ly, lane, p, j, q);
+      cm = materialize_one<ORDER>(a, Ly, lane, p, j, q);
     }
+    w_min = min(w_min, cm);
+    w_max = max(w_max, (uint32_t)(key >> 32));
+    w_minh = min(w_minh, (uint32_t)(key >> 32));
+  }
+  if (lane == 0) { s_mc[wave] = w_min; s_mh[wave] = w_max; s_ml[wave] = w_minh; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    RankStat rs;
+    rs.min_cmax = INT_MAX;
+    rs.max_hi = 0;
+    rs.min_hi = 0xFFFFFFFFu;
+    for (int w = 0; w < kRankWaves; ++w) {
+      rs.min_cmax = min(rs.min_cmax, s_mc[w]);
+      rs.max_hi = max(rs.max_hi, s_mh[w]);
+      rs.min_hi = min(rs.min_hi, s_ml[w]);
+    }
+    a.rstat[blockIdx.x] = rs;
   }
 }
 
@@ -665,7 +970,7 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
     const int x = a.ex_order[q];
     const int p = x / a.B;
     const int j = a.ex_job[x];
-    materialize_one<kOrderErase>(a, Ly, lane, p, j, q);
+    (void)materialize_one<kOrderErase>(a, Ly, lane, p, j, q);
   }
 }
 
@@ -673,19 +978,33 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
 // Best complete child and its sequence, by walking the trail backwards.
 __global__ __launch_bounds__(256) void k_backtrack(LevelArgs a) {
   __shared__ int s_cnt[128];
+  __shared__ uint32_t s_e[4];
+  __shared__ u64 s_b[4];
   int p = a.bt_parent, j = a.bt_job;
   const int L = a.L, m = a.m;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (p < 0) {
-    const Totals t = reduce_bstat(a.bstat, a.nblk_score);
+    uint32_t ex = 0;
+    u64 best = ~0ull;
+    for (int i = threadIdx.x; i < a.nblk_score; i += blockDim.x) {
+      ex += a.bstat[i].expanded;
+      best = a.bstat[i].best < best ? a.bstat[i].best : best;
+    }
+    ex = wave_sum(ex);
+    best = wave_min64(best);
+    if (lane == 0) { s_e[wave] = ex; s_b[wave] = best; }
+    __syncthreads();
     if (threadIdx.x != 0) return;
-    *a.expanded += (u64)t.expanded;
-    a.lvl_exp[L - 1] = t.expanded;
-    if (t.best == ~0ull) {
+    ex = 0; best = ~0ull;
+    for (int w = 0; w < 4; ++w) { ex += s_e[w]; best = s_b[w] < best ? s_b[w] : best; }
+    *a.expanded += (u64)ex;
+    a.lvl_exp[L - 1] = ex;
+    if (best == ~0ull) {
       a.result[0] = a.best_known; a.result[1] = 0; a.result[2] = 0;
       return;
     }
-    const uint32_t x = (uint32_t)t.best;
-    const int cmax = (int)(t.best >> 32);
+    const uint32_t x = (uint32_t)best;
+    const int cmax = (int)(best >> 32);
     p = (int)(x / (uint32_t)a.B);
     j = a.cjob[x];
     a.result[0] = min(cmax, a.best_known);
@@ -730,29 +1049,20 @@ hipError_t launch_score(const LevelArgs& a, int mode, bool single, hipStream_t s
   return hipGetLastError();
 }
 
-hipError_t launch_hist(const LevelArgs& a, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_hist, dim3(nblk), dim3(1024), 0, s, a);
+hipError_t launch_filter(const LevelArgs& a, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_filter, dim3(nblk), dim3(kFilterThreads), (size_t)a.bm_words * 4, s, a);
   return hipGetLastError();
 }
 
-hipError_t launch_partition(const LevelArgs& a, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_partition, dim3(nblk), dim3(1024), 0, s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_tie(const LevelArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(k_tie, dim3(1), dim3(1024), 0, s, a);
-  return hipGetLastError();
-}
-
-hipError_t launch_rank(const LevelArgs& a, int order, bool single, int nblk, hipStream_t s) {
-  const dim3 g(nblk), b(64 * kRankWaves);
+hipError_t launch_rank(const LevelArgs& a, int order, bool single, hipStream_t s) {
+  const dim3 g(a.nblk_rank), b(64 * kRankWaves);
+  const size_t lds = a.K <= kRankLdsMax ? (size_t)a.K * sizeof(u64) : 0;
   if (order == kOrderErase) {
-    if (single) hipLaunchKernelGGL((k_rank<kOrderErase, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_rank<kOrderErase, false>), g, b, 0, s, a);
+    if (single) hipLaunchKernelGGL((k_rank<kOrderErase, true>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_rank<kOrderErase, false>), g, b, lds, s, a);
   } else {
-    if (single) hipLaunchKernelGGL((k_rank<kOrderSwap, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_rank<kOrderSwap, false>), g, b, 0, s, a);
+    if (single) hipLaunchKernelGGL((k_rank<kOrderSwap, true>), g, b, lds, s, a);
+    else hipLaunchKernelGGL((k_rank<kOrderSwap, false>), g, b, lds, s, a);
   }
   return hipGetLastError();
 }

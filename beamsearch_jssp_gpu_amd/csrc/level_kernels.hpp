@@ -6,28 +6,48 @@
 
 namespace jssp {
 
-// Histogram width of the first selection digit.
+// First selection digit, kNB bins.  Exact form (when a lower makespan always
+// scores lower): 8 * (cmax - cmin) + min(noise offset in ulps, 7).  Generic
+// form: the score bits above a proven lower bound, shifted so the previous
+// level's survivor spread covers ~2^kSpreadBins bins.
 constexpr int kNB = 2048;
-constexpr int kLog2NB = 11;
-// Largest tie bucket sorted in LDS by the tie kernel.
-constexpr int kTieLds = 8192;
+// The shift makes the previous level's survivor spread cover ~2^10 bins.
+constexpr int kSpreadBins = 10;
+// Shift used at level 0 (no previous level; at most n children anyway).
+constexpr uint32_t kLevel0Shift = 20;
 // Parents per score block (one wave per parent, 1024-thread blocks).
 constexpr int kScoreWaves = 16;
-// Survivors per rank block (one wave per survivor, 256-thread blocks).
-constexpr int kRankWaves = 4;
+// Waves per filter / rank block.
+constexpr int kFilterThreads = 1024;
+constexpr int kRankWaves = 16;
+// Survivors ranked in LDS (8 bytes each) up to this beam width; wider beams
+// rank from global memory.
+constexpr int kRankLdsMax = 16384;
+// LDS bitmap over child indices for the threshold cut: up to 2^20 children.
+constexpr uint32_t kBitmapWordsMax = 32768;
 
 struct BlockStat {
-  uint32_t valid;        // kept children (score phase)
-  uint32_t minhi;        // min score bits among them
-  uint32_t maxhi;        // max score bits among them
-  uint32_t expanded;     // children evaluated
+  uint32_t valid;           // kept children
+  uint32_t expanded;        // children evaluated
   unsigned long long best;  // last level: min (cmax << 32 | child)
 };
 
+struct RankStat {
+  int min_cmax;     // smallest cmax among the survivors this block materialised
+  uint32_t max_hi;  // largest score bits among them
+  uint32_t min_hi;  // smallest score bits among them
+  uint32_t pad;
+};
+
 struct SelState {
-  uint32_t V, lo, hi, shift, Kp;  // valid count, score-bit range, digit shift, survivors
-  uint32_t T, cumlt, need, t;      // threshold bin, keys below it, survivors taken from it, its size
-  uint32_t pad[7];
+  uint32_t lo, s1;          // digit parameters of the level (written by score block 0)
+  uint32_t V, Kp;           // kept children, survivors = min(V, K)
+  uint32_t T1, cum_lt, t1, need;  // threshold bin, keys below it, its size, taken from it
+  uint32_t sure_ctr, tie_ctr, done_ctr;  // filter counters, zeroed before the level
+  uint32_t exact;           // 1: the digit is the exact (cmax, noise offset) digit
+  uint32_t single;          // the threshold bin holds a single score value
+  uint32_t sorted_from;     // cand[sorted_from, Kp) is already in rank order
+  uint32_t pad[2];
 };
 
 struct LevelArgs {
@@ -35,16 +55,20 @@ struct LevelArgs {
   const int* beam_cur;       // P x S
   int* beam_next;            // K x S
   int* cbuf;                 // single-phase children, (K*B) x S
-  unsigned long long* keys;  // K*B (also tie scratch)
+  unsigned long long* keys;  // K*B (also the refinement scratch)
   uint8_t* cjob;             // K*B: job of each child slot
-  unsigned long long* bucketed;
+  unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
+  unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut
+  uint16_t* kdig;            // K*B: first selection digit of each key
+  uint32_t* tbits;           // ceil(K*B/32) + 2: bitmap of the keys in a single-valued threshold bin
   uint32_t* hist;            // kNB, this level
-  uint32_t* fill;            // kNB, this level
   uint32_t* hist_next;       // kNB, zeroed for the next level
-  uint32_t* fill_next;
-  uint32_t* excl;            // kNB exclusive prefix of hist
   BlockStat* bstat;          // nblk_score partials
-  SelState* sel;
+  RankStat* rstat;           // nblk_rank partials of the previous level's survivors
+  SelState* sel;             // this level
+  const SelState* sel_prev;  // previous level
+  SelState* sel_next;        // next level (counters zeroed by the rank kernel)
+  SelState* sel_log;         // L entries or null: copy of each level's selection state
   int* pcount;               // L+1 parent counts
   int* trail;                // (L-1) x K
   unsigned long long* expanded;  // running total
@@ -58,7 +82,9 @@ struct LevelArgs {
   int* ex_rcount;            // K
   const int* ex_order;       // survivors (child index) in heap-array order
   int n, m, S, B, K, L;
-  int level, best_known, nblk_score;
+  int level, best_known, nblk_score, nblk_rank;
+  uint32_t bm_words;         // LDS bitmap words of the filter's threshold cut (dynamic LDS)
+  int exact_digit;           // use the exact (cmax, noise offset) digit (see k_score)
   uint32_t lseed;
   int bt_parent, bt_job;     // explicit backtrack start (exact modes), -1 = use bstat best
 };
@@ -66,10 +92,8 @@ struct LevelArgs {
 // mode: 1 ICH_DET, 2 PMS_GPU.  order: kOrderErase / kOrderSwap.
 hipError_t launch_init(const LevelArgs& a, hipStream_t s);
 hipError_t launch_score(const LevelArgs& a, int mode, bool single, hipStream_t s);
-hipError_t launch_hist(const LevelArgs& a, int nblk, hipStream_t s);
-hipError_t launch_partition(const LevelArgs& a, int nblk, hipStream_t s);
-hipError_t launch_tie(const LevelArgs& a, hipStream_t s);
-hipError_t launch_rank(const LevelArgs& a, int order, bool single, int nblk, hipStream_t s);
+hipError_t launch_filter(const LevelArgs& a, int nblk, hipStream_t s);
+hipError_t launch_rank(const LevelArgs& a, int order, bool single, hipStream_t s);
 hipError_t launch_exact_expand(const LevelArgs& a, int P, hipStream_t s);
 hipError_t launch_exact_materialize(const LevelArgs& a, int count, hipStream_t s);
 hipError_t launch_backtrack(const LevelArgs& a, hipStream_t s);

@@ -68,10 +68,15 @@ JSSP_HD uint32_t noise_hash(uint32_t seed, uint32_t cid, uint32_t bid) {
 // FP32 score, gpu/v1/solver.cu:121-127: base = cmax / ns (ns >= 1 after a
 // child is scheduled), plus (h % 1000) / 1e9.  Division is IEEE correctly
 // rounded on both sides (no fast-math, no reciprocal).
-JSSP_HD float gpu_score(int cmax, int ns, uint32_t h) {
-  float base = (ns == 0) ? (float)cmax : ((float)cmax / (float)ns);
+JSSP_HD float gpu_score_base(int cmax, int ns) {
+  return (ns == 0) ? (float)cmax : ((float)cmax / (float)ns);
+}
+JSSP_HD float gpu_score_add_noise(float base, uint32_t h) {
   float noise = (float)(h % 1000u) / 1000000000.0f;
   return base + noise;
+}
+JSSP_HD float gpu_score(int cmax, int ns, uint32_t h) {
+  return gpu_score_add_noise(gpu_score_base(cmax, ns), h);
 }
 
 }  // namespace jssp

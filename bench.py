@@ -29,16 +29,15 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-KIND_NAMES = ["expand_score", "select_histogram", "select_partition", "select_threshold_sort",
-              "rank_materialize"]
-KERNEL_SYMBOLS = ["k_score", "k_hist", "k_partition", "k_tie", "k_rank"]
+KIND_NAMES = ["expand_score", "select_filter", "rank_materialize"]
+KERNEL_SYMBOLS = ["k_score", "k_filter", "k_rank"]
 
 
 def roofline(eng, inst, K, mode, phases, seed):
     eng.set_profiling(True)
     eng.solve(inst, K, mode=mode, phases=phases, seed=seed)
     kinds = []
-    for k in range(5):
+    for k in range(len(KIND_NAMES)):
         ms, n, b = eng.kernel_time(k)
         kinds.append((k, ms, n, b))
     eng.set_profiling(False)

@@ -70,6 +70,8 @@ typedef enum jssp_mode {
 
 /* Flags for jssp_params.flags */
 #define JSSP_FLAG_GLOBAL_RAND 0x1u  /* exact modes: do not srand(seed); continue the process-global rand() stream */
+#define JSSP_FLAG_RADIX_CUT 0x2u    /* diagnostics: cut the selection threshold bin with the generic radix select only */
+#define JSSP_FLAG_GENERIC_DIGIT 0x4u /* diagnostics: use the generic score-bit selection digit only */
 
 typedef struct jssp_instance {
   int32_t n_jobs;
@@ -129,13 +131,18 @@ int jssp_solve_batch(jssp_engine* engine, int32_t count, const jssp_instance* in
 
 /* Per-kernel timing accumulated over the solves run since profiling was
  * switched on (jssp_engine_set_profiling(engine, 1) also resets it), for
- * roofline reporting.  kind: 0 = expand/score, 1 = selection histogram,
- * 2 = partition, 3 = threshold-bin sort, 4 = rank + materialise/gather.
+ * roofline reporting.  kind: 0 = expand/score (+ first selection
+ * histogram), 1 = selection filter, 2 = rank + materialise/gather.
  * total_ms is the sum of HIP-event durations around each launch on the
  * engine's stream; algorithmic_bytes the bytes the algorithm must move. */
 int jssp_engine_set_profiling(jssp_engine* engine, int32_t on);
 int jssp_engine_kernel_time(const jssp_engine* engine, int32_t kind, double* total_ms,
                             int64_t* launches, double* algorithmic_bytes);
+/* Selection diagnostics of the last profiled single-beam device solve, per
+ * level: out[0..9] = kept children, survivors, score-bit lower bound, digit
+ * shift, threshold bin, keys below it, keys in it, survivors taken from it,
+ * sure / tie counters. */
+int jssp_engine_level_stats(const jssp_engine* engine, int32_t level, uint32_t* out, int32_t count);
 
 /* ---- serving: continuous batching with HBM-budget admission ------------ */
 typedef struct jssp_server jssp_server;

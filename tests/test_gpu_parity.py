@@ -18,11 +18,12 @@ def _binst(i: Inst) -> bs.JSSPInstance:
     return bs.JSSPInstance(i.machine, i.duration, i.name)
 
 
-def _check_det(engine, inst, K, mode, phases, seed, bk=bs.NO_MAKESPAN):
-    print(f"case {inst.name} n={inst.n} m={inst.m} K={K} mode={mode} phases={phases} seed={seed} bk={bk}",
-          flush=True)
+def _check_det(engine, inst, K, mode, phases, seed, bk=bs.NO_MAKESPAN, flags=0):
+    print(f"case {inst.name} n={inst.n} m={inst.m} K={K} mode={mode} phases={phases} seed={seed} bk={bk} "
+          f"flags={flags}", flush=True)
     want = orc_gpu(inst, K, mode, bk, seed)
-    got = engine.solve(_binst(inst), K, mode=mode, phases=phases, best_known=bk, seed=seed, trace=True)
+    got = engine.solve(_binst(inst), K, mode=mode, phases=phases, best_known=bk, seed=seed, trace=True,
+                       flags=flags)
     assert got.makespan == want["makespan"]
     assert got.found == want["found"]
     assert got.expanded_states == want["expanded"]
@@ -52,6 +53,41 @@ def test_det_modes_10x5(engine, mode, phases):
 def test_det_modes_20x20_k1024(engine, mode, phases):
     inst = splitmix_instance(1, 20, 20, 1, 99)
     _check_det(engine, inst, 1024, mode, phases, seed=12345)
+
+
+SELECT_PATHS = [0, bs.FLAG_GENERIC_DIGIT, bs.FLAG_GENERIC_DIGIT | bs.FLAG_RADIX_CUT]
+
+
+@pytest.mark.parametrize("flags", SELECT_PATHS)
+@pytest.mark.parametrize("mode", [bs.MODE_ICH_DET, bs.MODE_PMS_GPU])
+def test_det_selection_paths(engine, mode, flags):
+    """Every selection path (exact digit + index-bitmap cut; generic digit +
+    exact score histograms; generic digit + radix cut) gives the oracle's beams."""
+    for s in (5, 6):
+        inst = splitmix_instance(s, 30, 10, 1, 99)
+        _check_det(engine, inst, 512, mode, 2, seed=s, flags=flags)
+        _check_det(engine, inst, 64, mode, 1, seed=s + 100, flags=flags)
+
+
+def test_det_equal_durations(engine):
+    """Unit durations: massive score ties, resolved by child index only."""
+    inst = splitmix_instance(3, 12, 6, 1, 1)
+    inst.name = "unit12x6"
+    for mode in (bs.MODE_ICH_DET, bs.MODE_PMS_GPU):
+        for flags in SELECT_PATHS:
+            _check_det(engine, inst, 300, mode, 2, seed=1, flags=flags)
+
+
+def test_det_zero_and_wide_durations(engine):
+    """Zero durations (scores are pure noise) and durations up to 10^6 (a
+    lower makespan no longer always scores lower, so the generic digit is used)."""
+    rng = np.random.default_rng(21)
+    z = Inst(np.array([rng.permutation(4) for _ in range(6)], np.int32), np.zeros((6, 4), np.int32), "zero6x4")
+    w = random_instance(rng, 10, 6, dmin=1, dmax=1_000_000)
+    w.name = "wide10x6"
+    for inst in (z, w):
+        for mode in (bs.MODE_ICH_DET, bs.MODE_PMS_GPU):
+            _check_det(engine, inst, 200, mode, 2, seed=3)
 
 
 @pytest.mark.parametrize("mode", [bs.MODE_ICH_DET, bs.MODE_PMS_GPU])
