@@ -25,7 +25,7 @@ __all__ = [
     "JSSPInstance", "Result", "Engine", "JsspError", "load_library", "library_path",
     "generate_instance", "parse_instance", "resolve_instance_file", "validate_sequence",
     "lower_bound", "beam_search_ich", "beam_search_pms", "run_gpu_beam_search_v1",
-    "run_gpu_beam_search_v2", "default_engine",
+    "run_gpu_beam_search_v2", "default_engine", "Server",
 ]
 
 NO_MAKESPAN = 500000
@@ -85,6 +85,10 @@ _SIGNATURES = {
                                         ctypes.POINTER(_Result), _I32P, _I32P]),
     "jssp_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_Instance),
                                         ctypes.POINTER(_Params), ctypes.POINTER(_Result)]),
+    "jssp_solve_batch_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_Instance),
+                                           ctypes.POINTER(_Params), ctypes.POINTER(_Result), ctypes.c_int32]),
+    "jssp_engine_set_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p]),
+    "jssp_engine_tuned_lanes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "jssp_engine_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "jssp_engine_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
@@ -232,7 +236,10 @@ class Engine:
         return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
                       int(r.levels), float(r.device_ms), int(r.best_seed_index), trail, pcount)
 
-    def solve_batch(self, insts: Sequence[JSSPInstance], params: Sequence[dict]) -> list[Result]:
+    def solve_batch(self, insts: Sequence[JSSPInstance], params: Sequence[dict],
+                    use_graph: bool = True) -> list[Result]:
+        """Batched solve; params[i] holds keyword arguments of _params
+        (beam_width, mode, phases, best_known, seed)."""
         n = len(insts)
         cis = (_Instance * n)(*[i._c() for i in insts])
         prs = (_Params * n)(*[_params(**p) for p in params])
@@ -242,9 +249,16 @@ class Engine:
             s = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
             seqs.append(s)
             rs[i].sequence = _p32(s)
-        self._check(self.lib.jssp_solve_batch(self.h, n, cis, prs, rs))
+        self._check(self.lib.jssp_solve_batch_ex(self.h, n, cis, prs, rs, 1 if use_graph else 0))
         return [Result(int(r.makespan), bool(r.found), seqs[i] if r.found else None, int(r.expanded_states),
                        int(r.levels), float(r.device_ms), int(r.best_seed_index)) for i, r in enumerate(rs)]
+
+    def set_autotune(self, on: bool, table_path: Optional[str] = None):
+        self._check(self.lib.jssp_engine_set_autotune(self.h, 1 if on else 0,
+                                                      table_path.encode() if table_path else None))
+
+    def tuned_lanes(self, n_jobs: int, n_machines: int, beam_width: int) -> int:
+        return int(self.lib.jssp_engine_tuned_lanes(self.h, n_jobs, n_machines, beam_width))
 
     def set_profiling(self, on: bool):
         self._check(self.lib.jssp_engine_set_profiling(self.h, 1 if on else 0))
@@ -354,3 +368,66 @@ def run_gpu_beam_search_v2(inst: JSSPInstance, beam_width: int, best_known: int,
                            out_best_sequence: Optional[np.ndarray] = None) -> int:
     """gpu/v2_twophases/solver.cu:256-386 (two-phase)."""
     return _gpu(inst, beam_width, best_known, random_seed, out_best_sequence, 2)
+
+
+class Server:
+    """Continuous-batching solve server (jssp_server): requests are admitted
+    as slots and HBM budget free up and advance together, one level per step."""
+
+    def __init__(self, engine: Engine, hbm_budget_bytes: int, max_slots: int = 64):
+        self.engine = engine
+        self.lib = engine.lib
+        cfg = _ServerConfig(int(hbm_budget_bytes), int(max_slots), 0)
+        h = ctypes.c_void_p()
+        engine._check(self.lib.jssp_server_create(engine.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.jssp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, inst: JSSPInstance, beam_width: int, mode: int = MODE_PMS_GPU, best_known: int = NO_MAKESPAN,
+               seed: int = 0) -> int:
+        ci = inst._c()
+        pr = _params(beam_width, mode, 2, best_known, seed)
+        t = ctypes.c_uint64()
+        self.engine._check(self.lib.jssp_serve_submit(self.h, ctypes.byref(ci), ctypes.byref(pr), ctypes.byref(t)))
+        self._keep[t.value] = inst
+        return int(t.value)
+
+    def step(self, levels: int = 16) -> int:
+        a = ctypes.c_int32()
+        self.engine._check(self.lib.jssp_serve_step(self.h, int(levels), ctypes.byref(a)))
+        return int(a.value)
+
+    def poll(self, ticket: int) -> Optional[Result]:
+        inst = self._keep[ticket]
+        seq = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
+        r = _Result()
+        r.sequence = _p32(seq)
+        rc = self.lib.jssp_serve_poll(self.h, ctypes.c_uint64(ticket), ctypes.byref(r))
+        if rc == 7:
+            return None
+        self.engine._check(rc)
+        return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                      int(r.levels), float(r.device_ms))
+
+    def cancel(self, ticket: int):
+        self.engine._check(self.lib.jssp_serve_cancel(self.h, ctypes.c_uint64(ticket)))
+
+    def drain(self):
+        self.engine._check(self.lib.jssp_serve_drain(self.h))
+
+    def times(self, ticket: int):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.engine._check(self.lib.jssp_serve_times(self.h, ctypes.c_uint64(ticket), ctypes.byref(a),
+                                                     ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value

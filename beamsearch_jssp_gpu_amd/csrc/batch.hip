@@ -1,19 +1,487 @@
-// batch.hip -- batched solves (placeholder: sequential loop over jssp_solve).
+// batch.hip -- many solves advanced together, one level per step.
+//
+// Every instance of a batch has its own workspace and descriptor; a step is
+// three launches (score, filter, rank) whose grids are the concatenation of
+// every instance's blocks (block map entry = instance, local block).  The
+// level lives on the device, so a step is the same launch sequence whatever
+// the instances' levels; finished instances' blocks return at once.  Steps
+// are captured once in a hipGraph and replayed.  The reference solves one
+// instance per call and one call per MPI rank (gpu/v1/main.cu:60-71); the
+// multi-seed driver below runs its seeds as one batch instead, sharing the
+// best bound through a device atomic.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <vector>
+
 #include "engine.hpp"
+#include "level_kernels.hpp"
+#include "workspace.hpp"
 
 namespace jssp {
-int solve_any(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
-              int32_t* trail_out, int32_t* pcount_out);
+
+void release_batch_state(jssp_engine* e) {
+  if (e->step_graph) (void)hipGraphExecDestroy(e->step_graph);
+  e->step_graph = nullptr;
+  e->step_graph_key.clear();
+  if (e->bbuf) (void)hipFree(e->bbuf);
+  e->bbuf = nullptr;
+  e->bcap = 0;
 }
 
-extern "C" int jssp_solve_batch(jssp_engine* e, int32_t count, const jssp_instance* insts,
-                                const jssp_params* params, jssp_result* results) {
-  if (!e || count < 0 || (count > 0 && (!insts || !params || !results))) return JSSP_ERR_INVALID;
-  std::lock_guard<std::recursive_mutex> lk(e->mu);
+namespace {
+
+constexpr int kStepsPerGraph = 16;
+
+struct BatchPlan {
+  int count = 0;
+  int Lmax = 0;
+  std::vector<Shape> shapes;
+  std::vector<Work> works;
+  std::vector<InstDesc> descs;
+  InstDesc* d_desc = nullptr;
+  BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_inst = nullptr;
+  int nms = 0, nmf = 0, nmr = 0;
+  size_t flds = 0, rlds = 0;
+  int* d_res = nullptr;                 // count x 4
+  unsigned long long* d_exp = nullptr;  // count
+  int* d_seq = nullptr;                 // sum of L
+  int* d_bound = nullptr;               // shared bound (multi-seed)
+  std::vector<size_t> seq_off;
+};
+
+int grow_batch_arena(jssp_engine* e, size_t bytes) {
+  if (bytes <= e->bcap) return JSSP_OK;
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  if (e->step_graph) {
+    (void)hipGraphExecDestroy(e->step_graph);
+    e->step_graph = nullptr;
+    e->step_graph_key.clear();
+  }
+  if (e->bbuf) JSSP_HIP(e, hipFree(e->bbuf));
+  e->bbuf = nullptr;
+  e->bcap = 0;
+  const size_t cap = bytes + bytes / 4;
+  if (hipMalloc(&e->bbuf, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    return set_error(e, JSSP_ERR_NOMEM, "hipMalloc", "batch arena of " + std::to_string(cap) + " bytes");
+  }
+  e->bcap = cap;
+  return JSSP_OK;
+}
+
+// Lays out and uploads a batch: packed op tables, per-instance workspaces,
+// descriptors, block maps and a shared result region.
+int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp_params* params,
+               const int* lanes, bool shared_bound, BatchPlan& bp) {
+  bp = BatchPlan();
+  bp.count = count;
+  std::vector<jssp_params> ps(params, params + count);
+  size_t ops_total = 0, seq_total = 0, work_total = 0;
+  std::vector<size_t> work_need(count);
+  std::vector<BlockRef> ms, mf, mr, mi;
   for (int i = 0; i < count; ++i) {
-    int rc = jssp::solve_any(e, &insts[i], &params[i], &results[i], nullptr, nullptr);
+    ps[i].phases = 2;  // batches materialise survivors only; beams are identical either way
+    Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0);
+    bp.shapes.push_back(s);
+    Work w;
+    work_need[i] = carve(s, nullptr, w);
+    work_total += work_need[i];
+    ops_total += (size_t)s.n * s.m;
+    bp.seq_off.push_back(seq_total);
+    seq_total += (size_t)s.L;
+    bp.Lmax = std::max(bp.Lmax, s.L);
+    append_maps(s, i, ms, mf, mr);
+    mi.push_back(make_int2(i, 0));
+    bp.flds = std::max(bp.flds, filter_lds_bytes(s, &ps[i]));
+    bp.rlds = std::max(bp.rlds, rank_lds_bytes(s));
+  }
+  Bump sz;
+  sz.take<int2>(ops_total);
+  sz.take<InstDesc>(count);
+  sz.take<BlockRef>(ms.size());
+  sz.take<BlockRef>(mf.size());
+  sz.take<BlockRef>(mr.size());
+  sz.take<BlockRef>(mi.size());
+  sz.take<int>((size_t)count * 4);
+  sz.take<unsigned long long>(count);
+  sz.take<int>(seq_total);
+  sz.take<int>(4);
+  const size_t head = sz.off + 256;
+  int rc = grow_batch_arena(e, head + work_total + 256 * (size_t)count);
+  if (rc) return rc;
+  Bump b;
+  b.base = e->bbuf;
+  int2* d_ops = b.take<int2>(ops_total);
+  bp.d_desc = b.take<InstDesc>(count);
+  bp.d_ms = b.take<BlockRef>(ms.size());
+  bp.d_mf = b.take<BlockRef>(mf.size());
+  bp.d_mr = b.take<BlockRef>(mr.size());
+  bp.d_inst = b.take<BlockRef>(mi.size());
+  bp.d_res = b.take<int>((size_t)count * 4);
+  bp.d_exp = b.take<unsigned long long>(count);
+  bp.d_seq = b.take<int>(seq_total);
+  bp.d_bound = b.take<int>(4);
+  size_t off = head;
+  size_t ops_off = 0;
+  for (int i = 0; i < count; ++i) {
+    Work w;
+    carve(bp.shapes[i], e->bbuf + off, w);
+    off += work_need[i];
+    off = (off + 255) & ~(size_t)255;
+    w.ops = d_ops + ops_off;
+    ops_off += (size_t)bp.shapes[i].n * bp.shapes[i].m;
+    w.result = bp.d_res + 4 * i;
+    w.expanded = bp.d_exp + i;
+    w.seq = bp.d_seq + bp.seq_off[i];
+    bp.works.push_back(w);
+    InstDesc d = make_desc(bp.shapes[i], w, &ps[i], &insts[i]);
+    d.shared_bound = shared_bound ? bp.d_bound : nullptr;
+    bp.descs.push_back(d);
+  }
+  bp.nms = (int)ms.size();
+  bp.nmf = (int)mf.size();
+  bp.nmr = (int)mr.size();
+  // One staging pass and one copy per region.
+  const size_t hbytes = ops_total * sizeof(int2) + (size_t)count * sizeof(InstDesc) +
+                        (ms.size() + mf.size() + mr.size() + mi.size()) * sizeof(BlockRef) + 64;
+  rc = ensure_host_staging(e, hbytes);
+  if (rc) return rc;
+  char* h = e->hbuf;
+  size_t hoff = 0;
+  auto put = [&](void* dst, const void* src, size_t bytes) -> int {
+    if (!bytes) return JSSP_OK;
+    std::memcpy(h + hoff, src, bytes);
+    JSSP_HIP(e, hipMemcpyAsync(dst, h + hoff, bytes, hipMemcpyHostToDevice, e->stream));
+    hoff += bytes;
+    return JSSP_OK;
+  };
+  {
+    int2* hops = reinterpret_cast<int2*>(h);
+    size_t k = 0;
+    for (int i = 0; i < count; ++i)
+      for (int q = 0; q < insts[i].n_jobs * insts[i].n_machines; ++q, ++k) {
+        hops[k].x = insts[i].machine[q];
+        hops[k].y = insts[i].duration[q];
+      }
+    JSSP_HIP(e, hipMemcpyAsync(d_ops, hops, ops_total * sizeof(int2), hipMemcpyHostToDevice, e->stream));
+    hoff = ops_total * sizeof(int2);
+  }
+  if ((rc = put(bp.d_desc, bp.descs.data(), (size_t)count * sizeof(InstDesc)))) return rc;
+  if ((rc = put(bp.d_ms, ms.data(), ms.size() * sizeof(BlockRef)))) return rc;
+  if ((rc = put(bp.d_mf, mf.data(), mf.size() * sizeof(BlockRef)))) return rc;
+  if ((rc = put(bp.d_mr, mr.data(), mr.size() * sizeof(BlockRef)))) return rc;
+  if ((rc = put(bp.d_inst, mi.data(), mi.size() * sizeof(BlockRef)))) return rc;
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  return JSSP_OK;
+}
+
+int enqueue_step(jssp_engine* e, const BatchPlan& bp) {
+  JSSP_HIP(e, launch_step_score(bp.d_desc, bp.d_ms, bp.nms, e->stream));
+  JSSP_HIP(e, launch_step_filter(bp.d_desc, bp.d_mf, bp.nmf, bp.flds, e->stream));
+  JSSP_HIP(e, launch_step_rank(bp.d_desc, bp.d_mr, bp.nmr, bp.rlds, e->stream));
+  return JSSP_OK;
+}
+
+// kStepsPerGraph steps captured once per batch layout and replayed.
+int enqueue_steps(jssp_engine* e, const BatchPlan& bp, int steps, bool use_graph) {
+  if (!use_graph) {
+    for (int t = 0; t < steps; ++t) {
+      int rc = enqueue_step(e, bp);
+      if (rc) return rc;
+    }
+    return JSSP_OK;
+  }
+  const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)bp.d_desc, (uint64_t)bp.nms, (uint64_t)bp.nmf,
+                                     (uint64_t)bp.nmr, (uint64_t)bp.flds, (uint64_t)bp.rlds,
+                                     (uint64_t)(uintptr_t)bp.d_ms, (uint64_t)(uintptr_t)bp.d_mf,
+                                     (uint64_t)(uintptr_t)bp.d_mr};
+  if (!e->step_graph || e->step_graph_key != key) {
+    if (e->step_graph) {
+      JSSP_HIP(e, hipGraphExecDestroy(e->step_graph));
+      e->step_graph = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    JSSP_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = JSSP_OK;
+    for (int t = 0; t < kStepsPerGraph && rc == JSSP_OK; ++t) rc = enqueue_step(e, bp);
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
     if (rc) return rc;
+    if (ce != hipSuccess) return set_error(e, JSSP_ERR_HIP, "hipStreamEndCapture", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&e->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      e->step_graph = nullptr;
+      return set_error(e, JSSP_ERR_HIP, "hipGraphInstantiate", hipGetErrorString(ie));
+    }
+    e->step_graph_key = key;
+  }
+  // Steps past an instance's last level are no-ops for it.
+  for (int t = 0; t < steps; t += kStepsPerGraph) JSSP_HIP(e, hipGraphLaunch(e->step_graph, e->stream));
+  return JSSP_OK;
+}
+
+int run_plan(jssp_engine* e, const BatchPlan& bp, bool use_graph) {
+  JSSP_HIP(e, launch_step_init(bp.d_desc, bp.d_inst, bp.count, e->stream));
+  int rc = enqueue_steps(e, bp, bp.Lmax, use_graph);
+  if (rc) return rc;
+  JSSP_HIP(e, launch_step_backtrack(bp.d_desc, bp.d_inst, bp.count, e->stream));
+  return JSSP_OK;
+}
+
+struct HostResults {
+  std::vector<int> res;
+  std::vector<unsigned long long> exp;
+  std::vector<int> seq;
+};
+
+int fetch_results(jssp_engine* e, const BatchPlan& bp, HostResults& hr) {
+  hr.res.resize((size_t)bp.count * 4);
+  hr.exp.resize(bp.count);
+  size_t seq_total = bp.seq_off.empty() ? 0 : bp.seq_off.back() + bp.shapes.back().L;
+  hr.seq.resize(seq_total);
+  JSSP_HIP(e, hipMemcpyAsync(hr.res.data(), bp.d_res, hr.res.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipMemcpyAsync(hr.exp.data(), bp.d_exp, hr.exp.size() * 8, hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipMemcpyAsync(hr.seq.data(), bp.d_seq, seq_total * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  return JSSP_OK;
+}
+
+// ---- lanes-per-parent tuning ------------------------------------------------
+uint64_t tune_key(int n, int m, int K) {
+  int kb = 0;
+  while ((1 << kb) < K) ++kb;  // K rounded up to a power of two
+  return ((uint64_t)n << 40) | ((uint64_t)m << 32) | (uint64_t)kb;
+}
+
+int lookup_tune(const jssp_engine* e, uint64_t key) {
+  for (const auto& kv : e->tune)
+    if (kv.first == key) return kv.second;
+  return 0;
+}
+
+void load_tune_file(jssp_engine* e) {
+  if (e->tune_path.empty()) return;
+  std::ifstream f(e->tune_path);
+  if (!f.good()) return;
+  // Format: one "n m kbits lanes" record per line.
+  int n, m, kb, g;
+  while (f >> n >> m >> kb >> g) {
+    const uint64_t key = ((uint64_t)n << 40) | ((uint64_t)m << 32) | (uint64_t)kb;
+    if (!lookup_tune(e, key) && (g == 16 || g == 32 || g == 64)) e->tune.push_back({key, g});
+  }
+}
+
+void save_tune_file(const jssp_engine* e) {
+  if (e->tune_path.empty()) return;
+  std::ofstream f(e->tune_path);
+  for (const auto& kv : e->tune)
+    f << (kv.first >> 40) << " " << ((kv.first >> 32) & 0xFF) << " " << (kv.first & 0xFFFFFFFF) << " "
+      << kv.second << "\n";
+}
+
+// For each (n, m, K bucket) class of the batch not yet in the table, times a
+// few levels of a small batch of its instances with every valid lanes-per-
+// parent shape and keeps the fastest.  Results do not depend on the shape.
+int autotune_classes(jssp_engine* e, int count, const jssp_instance* insts, const jssp_params* params) {
+  if (!e->autotune) return JSSP_OK;
+  if (e->tune.empty()) load_tune_file(e);
+  bool added = false;
+  std::vector<uint64_t> seen;
+  for (int i = 0; i < count; ++i) {
+    const int n = insts[i].n_jobs, m = insts[i].n_machines, K = params[i].beam_width;
+    const uint64_t key = tune_key(n, m, K);
+    if (lookup_tune(e, key) || std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
+    seen.push_back(key);
+    std::vector<int> members;
+    for (int k = i; k < count && (int)members.size() < 8; ++k)
+      if (tune_key(insts[k].n_jobs, insts[k].n_machines, params[k].beam_width) == key) members.push_back(k);
+    std::vector<jssp_instance> sub;
+    std::vector<jssp_params> subp;
+    for (int k : members) { sub.push_back(insts[k]); subp.push_back(params[k]); }
+    const int steps = std::min(24, n * m);
+    int best_g = 64;
+    double best_t = 1e30;
+    for (int g : {16, 32, 64}) {
+      if (g < 64 && g < n) continue;
+      std::vector<int> lanes(sub.size(), g);
+      BatchPlan bp;
+      int rc = plan_batch(e, (int)sub.size(), sub.data(), subp.data(), lanes.data(), false, bp);
+      if (rc) return rc;
+      JSSP_HIP(e, launch_step_init(bp.d_desc, bp.d_inst, bp.count, e->stream));
+      rc = enqueue_steps(e, bp, 2, false);  // warm up
+      if (rc) return rc;
+      JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+      rc = enqueue_steps(e, bp, steps, false);
+      if (rc) return rc;
+      JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+      JSSP_HIP(e, hipStreamSynchronize(e->stream));
+      float ms = 0.f;
+      JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+      if (ms < best_t) { best_t = ms; best_g = g; }
+    }
+    e->tune.push_back({key, best_g});
+    added = true;
+  }
+  if (added) save_tune_file(e);
+  return JSSP_OK;
+}
+
+int check_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp_params* params) {
+  for (int i = 0; i < count; ++i) {
+    int rc = check_instance(e, &insts[i]);
+    if (rc) return rc;
+    rc = check_params(e, &insts[i], &params[i]);
+    if (rc) return rc;
+    if (params[i].mode != JSSP_MODE_ICH_DET && params[i].mode != JSSP_MODE_PMS_GPU)
+      return set_error(e, JSSP_ERR_INVALID, "batch", "batched solves take modes ICH_DET and PMS_GPU");
+    if (params[i].n_seeds > 1 || params[i].n_passes > 1)
+      return set_error(e, JSSP_ERR_INVALID, "batch", "n_seeds / n_passes apply to jssp_solve");
   }
   return JSSP_OK;
 }
+
+}  // namespace
+
+// Batch of independent solves; results[i] equals jssp_solve(insts[i], params[i]).
+int solve_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp_params* params,
+                jssp_result* results, bool use_graph) {
+  int rc = check_batch(e, count, insts, params);
+  if (rc) return rc;
+  if (count == 0) return JSSP_OK;
+  JSSP_HIP(e, hipSetDevice(e->device));
+  rc = autotune_classes(e, count, insts, params);
+  if (rc) return rc;
+  std::vector<int> lanes(count);
+  for (int i = 0; i < count; ++i)
+    lanes[i] = lookup_tune(e, tune_key(insts[i].n_jobs, insts[i].n_machines, params[i].beam_width));
+  BatchPlan bp;
+  rc = plan_batch(e, count, insts, params, lanes.data(), false, bp);
+  if (rc) return rc;
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  rc = run_plan(e, bp, use_graph);
+  if (rc) return rc;
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  HostResults hr;
+  rc = fetch_results(e, bp, hr);
+  if (rc) return rc;
+  float ms = 0.f;
+  JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+  for (int i = 0; i < count; ++i) {
+    jssp_result& r = results[i];
+    r.makespan = hr.res[4 * i];
+    r.found = hr.res[4 * i + 1];
+    r.expanded_states = (int64_t)hr.exp[i];
+    r.levels = bp.shapes[i].L;
+    r.best_seed_index = 0;
+    r.device_ms = ms;
+    if (r.found && r.sequence)
+      std::memcpy(r.sequence, hr.seq.data() + bp.seq_off[i], (size_t)bp.shapes[i].L * sizeof(int));
+  }
+  return JSSP_OK;
+}
+
+// Multi-seed / multi-pass search (gpu/v1/main.cu:60-71, cpu/v1_pms/main.cpp:
+// 56-73 as one process on one GPU): each pass runs n_seeds independent beams
+// as one batch with seeds seed + pass * n_seeds + i; every beam of a pass
+// prunes with the device bound as it stood when the pass started, and each
+// beam that completes lowers it with atomicMin.  The result is the lowest
+// makespan found (ties: earliest pass, lowest seed index) -- the same as
+// running those solves one by one with the bound carried between passes.
+int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res) {
+  const int S = std::max(1, p->n_seeds), E = std::max(1, p->n_passes);
+  std::vector<jssp_instance> insts(S, *in);
+  std::vector<jssp_params> ps(S, *p);
+  for (auto& q : ps) { q.n_seeds = 1; q.n_passes = 1; }
+  BatchPlan bp;
+  int rc = plan_batch(e, S, insts.data(), ps.data(), nullptr, true, bp);
+  if (rc) return rc;
+  const int L = in->n_jobs * in->n_machines;
+  int bound = p->best_known;
+  JSSP_HIP(e, hipMemcpyAsync(bp.d_bound, &bound, sizeof(int), hipMemcpyHostToDevice, e->stream));
+  std::vector<HostResults> passes(E);
+  int64_t expanded = 0;
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  for (int ep = 0; ep < E; ++ep) {
+    if (ep > 0) {
+      for (int i = 0; i < S; ++i) bp.descs[i].base.lseed = p->seed + (uint32_t)(ep * S + i);
+      JSSP_HIP(e, hipStreamSynchronize(e->stream));
+      std::memcpy(e->hbuf, bp.descs.data(), (size_t)S * sizeof(InstDesc));
+      JSSP_HIP(e, hipMemcpyAsync(bp.d_desc, e->hbuf, (size_t)S * sizeof(InstDesc), hipMemcpyHostToDevice,
+                                 e->stream));
+    } else {
+      for (int i = 0; i < S; ++i) bp.descs[i].base.lseed = p->seed + (uint32_t)i;
+      JSSP_HIP(e, hipStreamSynchronize(e->stream));
+      std::memcpy(e->hbuf, bp.descs.data(), (size_t)S * sizeof(InstDesc));
+      JSSP_HIP(e, hipMemcpyAsync(bp.d_desc, e->hbuf, (size_t)S * sizeof(InstDesc), hipMemcpyHostToDevice,
+                                 e->stream));
+    }
+    rc = run_plan(e, bp, true);
+    if (rc) return rc;
+    rc = fetch_results(e, bp, passes[ep]);
+    if (rc) return rc;
+  }
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  float ms = 0.f;
+  JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+  int best = p->best_known, best_idx = -1;
+  const int* best_seq = nullptr;
+  for (int ep = 0; ep < E; ++ep)
+    for (int i = 0; i < S; ++i) {
+      const HostResults& hr = passes[ep];
+      expanded += (int64_t)hr.exp[i];
+      if (hr.res[4 * i + 1] && hr.res[4 * i + 2] < best) {
+        best = hr.res[4 * i + 2];
+        best_idx = ep * S + i;
+        best_seq = hr.seq.data() + bp.seq_off[i];
+      }
+    }
+  res->makespan = best;
+  res->found = best_idx >= 0 ? 1 : 0;
+  res->best_seed_index = std::max(0, best_idx);
+  res->expanded_states = expanded;
+  res->levels = L;
+  res->device_ms = ms;
+  if (res->found && res->sequence) std::memcpy(res->sequence, best_seq, (size_t)L * sizeof(int));
+  return JSSP_OK;
+}
+
+}  // namespace jssp
+
+extern "C" {
+
+int jssp_solve_batch(jssp_engine* e, int32_t count, const jssp_instance* insts, const jssp_params* params,
+                     jssp_result* results) {
+  if (!e || count < 0 || (count > 0 && (!insts || !params || !results))) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  return jssp::solve_batch(e, count, insts, params, results, true);
+}
+
+int jssp_solve_batch_ex(jssp_engine* e, int32_t count, const jssp_instance* insts, const jssp_params* params,
+                        jssp_result* results, int32_t use_graph) {
+  if (!e || count < 0 || (count > 0 && (!insts || !params || !results))) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  return jssp::solve_batch(e, count, insts, params, results, use_graph != 0);
+}
+
+int jssp_engine_set_autotune(jssp_engine* e, int32_t on, const char* table_path) {
+  if (!e) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  e->autotune = on != 0;
+  e->tune_path = table_path ? table_path : "";
+  e->tune.clear();
+  return JSSP_OK;
+}
+
+int jssp_engine_tuned_lanes(const jssp_engine* e, int32_t n_jobs, int32_t n_machines, int32_t beam_width) {
+  if (!e) return 0;
+  return jssp::lookup_tune(e, jssp::tune_key(n_jobs, n_machines, beam_width));
+}
+
+}  // extern "C"

@@ -6,9 +6,10 @@
 // beam_search_ich (cpu/v0_ich/solver.cpp:8-54), beam_search_pms
 // (cpu/v1_pms/solver.cpp:8-70).  Differences by design: the device arena and
 // stream live in the engine and are reused across solves (the reference
-// mallocs and frees nine buffers per call), the level loop enqueues a fixed
-// launch sequence with every count kept on the device (the reference sizes
-// grids from min(P*n, K) on the host), and errors are returned, never exit().
+// mallocs and frees nine buffers per call), the level loop enqueues the same
+// three launches at every level with every count and the level itself kept
+// on the device (the reference sizes grids from min(P*n, K) on the host), and
+// errors are returned, never exit().
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +22,7 @@
 #include "engine.hpp"
 #include "level_kernels.hpp"
 #include "snapshot.hpp"
+#include "workspace.hpp"
 
 namespace jssp {
 
@@ -96,44 +98,11 @@ int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p) 
   return JSSP_OK;
 }
 
-namespace {
+// ---------------------------------------------------------------------------
+// Workspace planning.
+int default_lanes(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
-// Device workspace of one single-instance solve.
-struct Work {
-  int2* ops;
-  int* beam[2];
-  int* cbuf;
-  unsigned long long* keys;
-  uint8_t* cjob;
-  unsigned long long* cand;
-  unsigned long long* ref_buf;
-  uint16_t* kdig;
-  uint32_t* tbits;
-  uint32_t* hist[2];
-  BlockStat* bstat;
-  RankStat* rstat;
-  SelState* sel[2];
-  int* pcount;
-  int* trail;
-  unsigned long long* expanded;
-  uint32_t* lvl_exp;
-  SelState* sel_log;
-  int* seq;
-  int* result;
-  int* ex_cmax;
-  uint8_t* ex_job;
-  uint8_t* ex_mc;
-  int* ex_rcount;
-  int* ex_order;
-};
-
-struct Shape {
-  int n, m, S, B, K, L;
-  bool single, exact;
-  int nblk_score, nblk_filter, nblk_rank;
-};
-
-Shape make_shape(const jssp_instance* in, const jssp_params* p) {
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int G) {
   Shape s;
   s.n = in->n_jobs; s.m = in->n_machines;
   s.S = 2 * s.n + s.m + 1;
@@ -142,9 +111,13 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p) {
   s.L = s.n * s.m;
   s.exact = (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT);
   s.single = !s.exact && p->phases == 1;
+  if (G != 16 && G != 32 && G != 64) G = default_lanes(s.n);
+  if (G < s.n || s.single) G = 64;
+  s.G = G;
   const long long N = (long long)s.K * s.B;
-  s.nblk_score = (int)std::max(1, std::min((s.K + kScoreWaves - 1) / kScoreWaves, 512));
-  s.nblk_filter = (int)std::max(1ll, std::min((N + 4095) / 4096, 256ll));
+  const int parents_per_block = kScoreWaves * (64 / s.G);
+  s.nblk_score = (int)std::max(1, std::min((s.K + parents_per_block - 1) / parents_per_block, 512));
+  s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads, 512ll));
   s.nblk_rank = (int)std::max(1, std::min((s.K + kRankWaves - 1) / kRankWaves, 256));
   return s;
 }
@@ -188,8 +161,6 @@ size_t carve(const Shape& s, char* base, Work& w) {
   return b.off + 256;
 }
 
-}  // namespace
-
 // The exact selection digit is valid when, at every level, a lower makespan
 // always gives a lower FP32 score: for cmax c1 < c2 at ns scheduled ops,
 // score(c1) <= c1/ns + 1e-6 + ulp and score(c2) >= c2/ns - ulp/2, so it
@@ -206,8 +177,6 @@ bool exact_digit_applies(const jssp_instance* in) {
   const double L = (double)in->n_jobs * in->n_machines;
   return 1.0 / L > 1e-6 + 2.0 * ulp;
 }
-
-namespace {
 
 LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   LevelArgs a;
@@ -229,11 +198,11 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.sel = w.sel[0];
   a.sel_prev = w.sel[1];
   a.sel_next = w.sel[1];
+  a.sel_log = nullptr;
   a.pcount = w.pcount;
   a.trail = w.trail;
   a.expanded = w.expanded;
   a.lvl_exp = w.lvl_exp;
-  a.sel_log = nullptr;
   a.seq_out = w.seq;
   a.result = w.result;
   a.ex_cmax = w.ex_cmax; a.ex_job = w.ex_job; a.ex_mc = w.ex_mc;
@@ -247,57 +216,39 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
                : (uint32_t)std::min<size_t>(((size_t)s.K * s.B + 31) / 32, kBitmapWordsMax);
   a.lseed = p->seed;
   a.bt_parent = -1; a.bt_job = -1;
+  a.mode = (p->mode == JSSP_MODE_PMS_GPU) ? 2 : 1;
+  a.G = s.G;
   return a;
 }
 
-void set_level(LevelArgs& a, const Work& w, int level, uint32_t seed) {
-  a.level = level;
-  a.lseed = seed + (uint32_t)level;
-  a.beam_cur = w.beam[level & 1];
-  a.beam_next = w.beam[(level + 1) & 1];
-  a.hist = w.hist[level & 1];
-  a.hist_next = w.hist[(level + 1) & 1];
-  a.sel = w.sel[level & 1];
-  a.sel_prev = w.sel[(level + 1) & 1];
-  a.sel_next = w.sel[(level + 1) & 1];
+InstDesc make_desc(const Shape& s, const Work& w, const jssp_params* p, const jssp_instance* in) {
+  InstDesc d;
+  std::memset(&d, 0, sizeof(d));
+  d.base = base_args(s, w, p);
+  d.base.exact_digit = (!(p->flags & JSSP_FLAG_GENERIC_DIGIT) && exact_digit_applies(in)) ? 1 : 0;
+  d.beam[0] = w.beam[0]; d.beam[1] = w.beam[1];
+  d.hist[0] = w.hist[0]; d.hist[1] = w.hist[1];
+  d.sel[0] = w.sel[0]; d.sel[1] = w.sel[1];
+  d.nblk_score = s.nblk_score;
+  d.nblk_filter = s.nblk_filter;
+  d.nblk_rank = s.nblk_rank;
+  d.level = 0;
+  return d;
 }
 
-// Per-kernel HIP-event timing (optional).
-struct Prof {
-  jssp_engine* e;
-  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
-  size_t next = 0;
-  hipEvent_t get() {
-    if (next >= e->pev.size()) {
-      hipEvent_t ev;
-      if (hipEventCreate(&ev) != hipSuccess) return nullptr;
-      e->pev.push_back(ev);
-    }
-    return e->pev[next++];
-  }
-  hipEvent_t begin() {
-    if (!e->profiling) return nullptr;
-    hipEvent_t a = get();
-    if (a) (void)hipEventRecord(a, e->stream);
-    return a;
-  }
-  void end(int kind, hipEvent_t a) {
-    if (!e->profiling || !a) return;
-    hipEvent_t b = get();
-    if (!b) return;
-    (void)hipEventRecord(b, e->stream);
-    marks.push_back({kind, {a, b}});
-  }
-  void collect() {
-    for (auto& mk : marks) {
-      float ms = 0.f;
-      if (hipEventElapsedTime(&ms, mk.second.first, mk.second.second) == hipSuccess) {
-        e->kt[mk.first].ms += ms;
-        e->kt[mk.first].launches += 1;
-      }
-    }
-  }
-};
+void append_maps(const Shape& s, int inst, std::vector<BlockRef>& score, std::vector<BlockRef>& filter,
+                 std::vector<BlockRef>& rank) {
+  for (int b = 0; b < s.nblk_score; ++b) score.push_back(make_int2(inst, b));
+  for (int b = 0; b < s.nblk_filter; ++b) filter.push_back(make_int2(inst, b));
+  for (int b = 0; b < s.nblk_rank; ++b) rank.push_back(make_int2(inst, b));
+}
+
+size_t filter_lds_bytes(const Shape& s, const jssp_params* p) {
+  if (p->flags & JSSP_FLAG_RADIX_CUT) return 0;
+  return 4 * std::min<size_t>(((size_t)s.K * s.B + 31) / 32, kBitmapWordsMax);
+}
+
+size_t rank_lds_bytes(const Shape& s) { return s.K <= kRankLdsMax ? (size_t)s.K * 8 : 0; }
 
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops) {
   const int total = in->n_jobs * in->n_machines;
@@ -306,10 +257,43 @@ int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops) {
   int2* h = reinterpret_cast<int2*>(e->hbuf);
   for (int i = 0; i < total; ++i) { h[i].x = in->machine[i]; h[i].y = in->duration[i]; }
   JSSP_HIP(e, hipMemcpyAsync(d_ops, h, (size_t)total * sizeof(int2), hipMemcpyHostToDevice, e->stream));
-  // The staging buffer is reused below; finish the copy first.
+  // The staging buffer is reused right after; finish the copy first.
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
   return JSSP_OK;
 }
+
+hipEvent_t Prof::get() {
+  if (next >= e->pev.size()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    e->pev.push_back(ev);
+  }
+  return e->pev[next++];
+}
+hipEvent_t Prof::begin() {
+  if (!e->profiling) return nullptr;
+  hipEvent_t a = get();
+  if (a) (void)hipEventRecord(a, e->stream);
+  return a;
+}
+void Prof::end(int kind, hipEvent_t a) {
+  if (!e->profiling || !a) return;
+  hipEvent_t b = get();
+  if (!b) return;
+  (void)hipEventRecord(b, e->stream);
+  marks.push_back({kind, {a, b}});
+}
+void Prof::collect() {
+  for (auto& mk : marks) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, mk.second.first, mk.second.second) == hipSuccess) {
+      e->kt[mk.first].ms += ms;
+      e->kt[mk.first].launches += 1;
+    }
+  }
+}
+
+namespace {
 
 void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int>& pc,
                            const std::vector<uint32_t>& lexp) {
@@ -330,47 +314,70 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
   }
 }
 
-// PMS_GPU / ICH_DET, one beam.
-int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, uint32_t seed,
-                 int bound, jssp_result* res, int32_t* trail_out, int32_t* pcount_out) {
+// PMS_GPU / ICH_DET, one beam: a batch of one on the step kernels.
+int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, uint32_t seed, int bound,
+                 jssp_result* res, int32_t* trail_out, int32_t* pcount_out) {
   jssp_params q = *p;
   q.seed = seed;
   q.best_known = bound;
-  const Shape s = make_shape(in, &q);
+  const Shape s = make_shape(in, &q, 0);
   Work w;
+  Bump tail;
   const size_t need = carve(s, nullptr, w);
-  int rc = ensure_device_arena(e, need);
+  std::vector<BlockRef> ms, mf, mr;
+  append_maps(s, 0, ms, mf, mr);
+  const size_t extra = sizeof(InstDesc) + (ms.size() + mf.size() + mr.size() + 1) * sizeof(BlockRef) + 1024;
+  int rc = ensure_device_arena(e, need + extra);
   if (rc) return rc;
   carve(s, e->dbuf, w);
+  tail.base = e->dbuf + need;
+  InstDesc* d_desc = tail.take<InstDesc>(1);
+  BlockRef* d_ms = tail.take<BlockRef>(ms.size());
+  BlockRef* d_mf = tail.take<BlockRef>(mf.size());
+  BlockRef* d_mr = tail.take<BlockRef>(mr.size());
+  BlockRef* d_one = tail.take<BlockRef>(1);
   rc = upload_ops(e, in, w.ops);
   if (rc) return rc;
-  LevelArgs a = base_args(s, w, &q);
-  a.exact_digit = (!(q.flags & JSSP_FLAG_GENERIC_DIGIT) && exact_digit_applies(in)) ? 1 : 0;
-  if (e->profiling) a.sel_log = w.sel_log;
-  const int mode = (q.mode == JSSP_MODE_ICH_DET) ? 1 : 2;
-  const int order = (mode == 1) ? kOrderErase : kOrderSwap;
-  Prof prof{e, {}, 0};
-  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
-  JSSP_HIP(e, launch_init(a, e->stream));
-  for (int l = 0; l < s.L; ++l) {
-    set_level(a, w, l, seed);
-    hipEvent_t t0 = prof.begin();
-    JSSP_HIP(e, launch_score(a, mode, s.single, e->stream));
-    prof.end(kKindScore, t0);
-    if (l == s.L - 1) break;
-    t0 = prof.begin();
-    JSSP_HIP(e, launch_filter(a, s.nblk_filter, e->stream));
-    prof.end(kKindFilter, t0);
-    t0 = prof.begin();
-    JSSP_HIP(e, launch_rank(a, order, s.single, e->stream));
-    prof.end(kKindRank, t0);
-  }
-  JSSP_HIP(e, launch_backtrack(a, e->stream));
-  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
-  // Results.
-  const size_t hb = 64 + (size_t)s.L * 4 * 3 + 16;
-  rc = ensure_host_staging(e, hb);
+  InstDesc desc = make_desc(s, w, &q, in);
+  if (e->profiling) desc.base.sel_log = w.sel_log;
+  // Descriptor and maps go up through the pinned staging buffer.
+  const size_t hb = sizeof(InstDesc) + (ms.size() + mf.size() + mr.size() + 1) * sizeof(BlockRef);
+  rc = ensure_host_staging(e, std::max(hb, (size_t)64 + (size_t)s.L * 4 + 64));
   if (rc) return rc;
+  {
+    char* h = e->hbuf;
+    size_t off = 0;
+    auto put = [&](void* dst, const void* src, size_t bytes) -> int {
+      std::memcpy(h + off, src, bytes);
+      JSSP_HIP(e, hipMemcpyAsync(dst, h + off, bytes, hipMemcpyHostToDevice, e->stream));
+      off += bytes;
+      return JSSP_OK;
+    };
+    const BlockRef one = make_int2(0, 0);
+    if ((rc = put(d_desc, &desc, sizeof(InstDesc)))) return rc;
+    if ((rc = put(d_ms, ms.data(), ms.size() * sizeof(BlockRef)))) return rc;
+    if ((rc = put(d_mf, mf.data(), mf.size() * sizeof(BlockRef)))) return rc;
+    if ((rc = put(d_mr, mr.data(), mr.size() * sizeof(BlockRef)))) return rc;
+    if ((rc = put(d_one, &one, sizeof(BlockRef)))) return rc;
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  }
+  const size_t flds = filter_lds_bytes(s, &q), rlds = rank_lds_bytes(s);
+  Prof prof(e);
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  JSSP_HIP(e, launch_step_init(d_desc, d_one, 1, e->stream));
+  for (int l = 0; l < s.L; ++l) {
+    hipEvent_t t0 = prof.begin();
+    JSSP_HIP(e, launch_step_score(d_desc, d_ms, (int)ms.size(), e->stream));
+    prof.end(kKindScore, t0);
+    t0 = prof.begin();
+    JSSP_HIP(e, launch_step_filter(d_desc, d_mf, (int)mf.size(), flds, e->stream));
+    if (l < s.L - 1) prof.end(kKindFilter, t0);
+    t0 = prof.begin();
+    JSSP_HIP(e, launch_step_rank(d_desc, d_mr, (int)mr.size(), rlds, e->stream));
+    if (l < s.L - 1) prof.end(kKindRank, t0);
+  }
+  JSSP_HIP(e, launch_step_backtrack(d_desc, d_one, 1, e->stream));
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
   int* h_res = reinterpret_cast<int*>(e->hbuf);
   unsigned long long* h_exp = reinterpret_cast<unsigned long long*>(e->hbuf + 32);
   int* h_seq = reinterpret_cast<int*>(e->hbuf + 64);
@@ -378,15 +385,14 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   JSSP_HIP(e, hipMemcpyAsync(h_exp, w.expanded, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->stream));
   JSSP_HIP(e, hipMemcpyAsync(h_seq, w.seq, (size_t)s.L * sizeof(int), hipMemcpyDeviceToHost, e->stream));
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
-  float ms = 0.f;
-  JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+  float msec = 0.f;
+  JSSP_HIP(e, hipEventElapsedTime(&msec, e->ev_start, e->ev_stop));
   res->makespan = h_res[0];
   res->found = h_res[1];
   res->expanded_states = (int64_t)*h_exp;
   res->levels = s.L;
-  res->device_ms = ms;
+  res->device_ms = msec;
   if (res->found && res->sequence) std::memcpy(res->sequence, h_seq, (size_t)s.L * sizeof(int));
-  if (!res->found && res->sequence) { /* untouched, as documented */ }
   if (e->profiling) {
     prof.collect();
     std::vector<int> pc(s.L + 1);
@@ -433,10 +439,16 @@ void heap_insert(std::vector<HeapEntry>& beam, const HeapEntry& h, int k) {
   }
 }
 
+void set_exact_level(LevelArgs& a, const Work& w, int level) {
+  a.level = level;
+  a.beam_cur = w.beam[level & 1];
+  a.beam_next = w.beam[(level + 1) & 1];
+}
+
 int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
                 int32_t* trail_out, int32_t* pcount_out) {
   const bool ich = (p->mode == JSSP_MODE_ICH_EXACT);
-  const Shape s = make_shape(in, p);
+  const Shape s = make_shape(in, p, 64);
   Work w;
   const size_t need = carve(s, nullptr, w);
   int rc = ensure_device_arena(e, need);
@@ -464,7 +476,7 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, j
   int best = JSSP_NO_MAKESPAN, best_p = -1, best_j = -1;
   int levels = 0;
   for (int l = 0; l < s.L && P > 0; ++l) {
-    set_level(a, w, l, 0);
+    set_exact_level(a, w, l);
     JSSP_HIP(e, launch_exact_expand(a, P, e->stream));
     JSSP_HIP(e, hipMemcpyAsync(h_r, w.ex_rcount, (size_t)P * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     JSSP_HIP(e, hipMemcpyAsync(h_cmax, w.ex_cmax, (size_t)P * s.B * sizeof(int), hipMemcpyDeviceToHost, e->stream));
@@ -546,10 +558,8 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, j
 
 }  // namespace
 
-// Multi-seed / multi-pass driver (gpu/v1/main.cu:60-71 and
-// cpu/v1_pms/main.cpp:56-73 as one process): every beam of a pass prunes with
-// the bound at the start of the pass; after the pass the bound is the
-// minimum.  Used by jssp_solve for n_seeds > 1 without batching.
+int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res);
+
 int solve_any(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
               int32_t* trail_out, int32_t* pcount_out) {
   int rc = check_instance(e, in);
@@ -561,42 +571,8 @@ int solve_any(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jss
   res->best_seed_index = 0;
   if (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT)
     return solve_exact(e, in, p, res, trail_out, pcount_out);
-  const int seeds = std::max(1, p->n_seeds);
-  const int passes = std::max(1, p->n_passes);
-  if (seeds == 1 && passes == 1)
-    return solve_device(e, in, p, p->seed, p->best_known, res, trail_out, pcount_out);
-  int bound = p->best_known;
-  jssp_result best;
-  std::memset(&best, 0, sizeof(best));
-  best.makespan = bound;
-  std::vector<int32_t> seq((size_t)in->n_jobs * in->n_machines);
-  int64_t expanded = 0;
-  double ms = 0.0;
-  for (int e_ = 0; e_ < passes; ++e_) {
-    const int pass_bound = bound;
-    for (int i = 0; i < seeds; ++i) {
-      jssp_result r;
-      std::memset(&r, 0, sizeof(r));
-      r.sequence = seq.data();
-      rc = solve_device(e, in, p, p->seed + (uint32_t)(e_ * seeds + i), pass_bound, &r, nullptr, nullptr);
-      if (rc) return rc;
-      expanded += r.expanded_states;
-      ms += r.device_ms;
-      if (r.found && r.makespan < bound) {
-        bound = r.makespan;
-        best = r;
-        best.best_seed_index = e_ * seeds + i;
-        if (res->sequence) std::memcpy(res->sequence, seq.data(), seq.size() * sizeof(int32_t));
-      }
-    }
-  }
-  res->makespan = bound;
-  res->found = best.found;
-  res->best_seed_index = best.best_seed_index;
-  res->expanded_states = expanded;
-  res->levels = in->n_jobs * in->n_machines;
-  res->device_ms = ms;
-  return JSSP_OK;
+  if (std::max(1, p->n_seeds) > 1 || std::max(1, p->n_passes) > 1) return solve_multiseed(e, in, p, res);
+  return solve_device(e, in, p, p->seed, p->best_known, res, trail_out, pcount_out);
 }
 
 }  // namespace jssp
@@ -628,6 +604,7 @@ void jssp_engine_destroy(jssp_engine* e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  jssp::release_batch_state(e);
   for (hipEvent_t ev : e->pev) (void)hipEventDestroy(ev);
   if (e->ev_start) (void)hipEventDestroy(e->ev_start);
   if (e->ev_stop) (void)hipEventDestroy(e->ev_stop);

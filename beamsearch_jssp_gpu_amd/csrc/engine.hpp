@@ -33,6 +33,15 @@ struct jssp_engine {
   std::vector<hipEvent_t> pev;  // event pool for per-kernel timing
   KernelTime kt[kNumKinds];
   std::vector<jssp::SelState> level_log;  // per-level selection state of the last profiled solve
+  // Batched solves: their own arena (kept apart from the single-solve one)
+  // and the captured step graph of the last batch layout.
+  char* bbuf = nullptr;
+  size_t bcap = 0;
+  hipGraphExec_t step_graph = nullptr;
+  std::vector<uint64_t> step_graph_key;
+  int autotune = 1;       // consult/extend the lanes-per-parent tuning table
+  std::string tune_path;  // JSON file of the tuning table ("" = in memory only)
+  std::vector<std::pair<uint64_t, int>> tune;  // (n, m, K bucket) -> lanes per parent
   std::recursive_mutex mu;
 };
 
@@ -57,6 +66,7 @@ int ensure_device_arena(jssp_engine* e, size_t bytes);
 int ensure_host_staging(jssp_engine* e, size_t bytes);
 int check_instance(jssp_engine* e, const jssp_instance* in);
 int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p);
+void release_batch_state(jssp_engine* e);
 
 }  // namespace jssp
 

@@ -205,31 +205,33 @@ __global__ void k_init(LevelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Score (two-phase) or expand (single-phase): one wave per parent.
-// MODE 1 = ICH_DET (every ready op, slot = ready position),
-// MODE 2 = PMS_GPU (ops on the hash-chosen pivot machine, gpu/v1/solver.cu:47-70).
-template <int MODE, bool SINGLE>
-__global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
+// Score (two-phase) or expand (single-phase).  A group of G lanes (G = 16,
+// 32 or 64, G >= n unless G = 64) handles one parent; lane j of the group
+// holds job j (and j + 64 when n > 64).
+// Mode 1 = ICH_DET (every ready op, slot = ready position),
+// mode 2 = PMS_GPU (ops on the hash-chosen pivot machine, gpu/v1/solver.cu:47-70).
+__device__ void score_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_h[kNB];
-  __shared__ uint32_t s_cnt[kScoreWaves], s_exp[kScoreWaves];
-  __shared__ u64 s_best[kScoreWaves];
-  __shared__ int s_rmin[kScoreWaves];
-  __shared__ uint32_t s_rmax[kScoreWaves], s_rlo[kScoreWaves];
+  __shared__ uint32_t s_cnt[16], s_exp[16];
+  __shared__ u64 s_best[16];
+  __shared__ int s_rmin[16];
+  __shared__ uint32_t s_rmax[16], s_rlo[16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nwaves = (int)(blockDim.x >> 6);
   const Layout Ly = make_layout(a.n, a.m);
   const int n = a.n, m = a.m, S = Ly.S, B = a.B;
   const int P = a.pcount[a.level];
   const bool last = (a.level == a.L - 1);
   const int ns = a.level + 1;
-  const int NH = (n + 63) >> 6;
   const int NW = (S + 63) >> 6;
-  constexpr int ORDER = (MODE == 1) ? kOrderErase : kOrderSwap;
+  const int mode = a.mode;
+  const bool single = a.cbuf != nullptr;
 
   // Digit parameters.  Every child's score is >= fl(cmin / ns), cmin = the
   // smallest parent cmax: cmax only grows, correctly rounded division is
   // monotone, and adding the non-negative noise cannot round below the base.
-  // The shift spreads the previous level's survivors' score range over
-  // ~2^kSpreadBins bins.
+  // The generic shift spreads the previous level's survivors' score range
+  // over ~2^kSpreadBins bins.
   uint32_t lo = 0, s1 = kLevel0Shift;
   if (a.level > 0 && !last) {
     int cmin = INT_MAX;
@@ -247,7 +249,7 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
     cmin = INT_MAX;
     mxh = 0;
     mnh = 0xFFFFFFFFu;
-    for (int w = 0; w < kScoreWaves; ++w) {
+    for (int w = 0; w < nwaves; ++w) {
       cmin = min(cmin, s_rmin[w]);
       mxh = max(mxh, s_rmax[w]);
       mnh = min(mnh, s_rlo[w]);
@@ -263,24 +265,33 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
   } else if (a.exact_digit) {
     lo = 0;  // the root's cmax
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) { a.sel->lo = lo; a.sel->s1 = s1; a.sel->exact = a.exact_digit ? 1u : 0u; }
+  if (local == 0 && threadIdx.x == 0) { a.sel->lo = lo; a.sel->s1 = s1; a.sel->exact = a.exact_digit ? 1u : 0u; }
   for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
   __syncthreads();
+
+  const int G = a.G;
+  const int lg = (G == 64) ? 6 : (G == 32 ? 5 : 4);
+  const int PPW = 64 >> lg;
+  const int sub = lane >> lg, gl = lane & (G - 1);
+  const u64 gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (sub * G));
+  const int NH = (G == 64) ? ((n + 63) >> 6) : 1;
 
   uint32_t w_cnt = 0, w_exp = 0;
   u64 w_best = ~0ull;
 
-  for (int p = blockIdx.x * kScoreWaves + wave; p < P; p += gridDim.x * kScoreWaves) {
-    const int* par = a.beam_cur + (size_t)p * S;
-    const int pc = par[Ly.off_cmax];
+  for (int pb = (local * nwaves + wave) * PPW; pb < P; pb += nblk * nwaves * PPW) {
+    const int p = pb + sub;
+    const bool pact = p < P;
+    const int* par = a.beam_cur + (size_t)(pact ? p : 0) * S;
+    const int pc = pact ? par[Ly.off_cmax] : 0;
     int pos[2], mc[2], f[2], cm[2], nxv[2];
     bool rdy[2];
     int r = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       rdy[h] = false; pos[h] = kFinishedPos; mc[h] = 0; f[h] = 0; cm[h] = 0; nxv[h] = 0;
-      const int j = h * 64 + lane;
-      if (h < NH && j < n) {
+      const int j = h * 64 + gl;
+      if (pact && h < NH && j < n) {
         const int w = par[Ly.off_jw + j];
         pos[h] = jw_pos(w);
         if (pos[h] != kFinishedPos) {
@@ -296,47 +307,51 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
           cm[h] = max(pc, f[h]);
         }
       }
-      r += __popcll(__ballot(rdy[h]));
+      r += __popcll(__ballot(rdy[h]) & gmask);
     }
 
     int slot[2];
     bool val[2];
     int nvalid;
-    if (MODE == 1) {
+    if (mode == 1) {
       slot[0] = pos[0]; slot[1] = pos[1];
       val[0] = rdy[0]; val[1] = rdy[1];
       nvalid = r;
     } else {
-      nvalid = 0;
-      val[0] = val[1] = false;
-      slot[0] = slot[1] = 0;
-      if (r > 0) {
-        const uint32_t hsh = pivot_hash(a.lseed + (uint32_t)p);
-        const int tpos = (int)(hsh % (uint32_t)r);
-        int tmc = -1;
+      // Pivot: ready[hash % r] (gpu/v1/solver.cu:51-56); every lane reads its
+      // own group's source lane.
+      const uint32_t hsh = pivot_hash(a.lseed + (uint32_t)p);
+      const int tpos = r > 0 ? (int)(hsh % (uint32_t)r) : -1;
+      const u64 b0 = __ballot(rdy[0] && pos[0] == tpos) & gmask;
+      const u64 b1 = __ballot(rdy[1] && pos[1] == tpos) & gmask;
+      const int t0 = __shfl(mc[0], b0 ? __ffsll((long long)b0) - 1 : lane);
+      const int t1 = __shfl(mc[1], b1 ? __ffsll((long long)b1) - 1 : lane);
+      const int tmc = b0 ? t0 : (b1 ? t1 : -1);
+      val[0] = rdy[0] && mc[0] == tmc;
+      val[1] = rdy[1] && mc[1] == tmc;
+      // Slot = rank of the ready position among the pivot machine's ops:
+      // the group's candidate positions as a 128-bit mask.
+      u64 plo = 0, phi = 0;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const u64 bm = __ballot(rdy[h] && pos[h] == tpos);
-          if (bm) tmc = __shfl(mc[h], __ffsll((long long)bm) - 1);
-        }
-        val[0] = rdy[0] && mc[0] == tmc;
-        val[1] = rdy[1] && mc[1] == tmc;
-        // Slot = rank of the ready position among the pivot machine's ops.
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          u64 bm = __ballot(val[h]);
-          nvalid += __popcll(bm);
-          while (bm) {
-            const int l = __ffsll((long long)bm) - 1;
-            bm &= bm - 1;
-            const int pl = __shfl(pos[h], l);
-            slot[0] += (pl < pos[0]) ? 1 : 0;
-            slot[1] += (pl < pos[1]) ? 1 : 0;
-          }
+      for (int h = 0; h < 2; ++h) {
+        if (val[h]) {
+          if (pos[h] < 64) plo |= 1ull << pos[h];
+          else phi |= 1ull << (pos[h] - 64);
         }
       }
+      for (int o = 1; o < G; o <<= 1) {
+        plo |= __shfl_xor(plo, o);
+        phi |= __shfl_xor(phi, o);
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int ps = pos[h];
+        slot[h] = ps < 64 ? __popcll(plo & ((1ull << ps) - 1ull))
+                          : __popcll(plo) + __popcll(phi & ((1ull << (ps - 64)) - 1ull));
+      }
+      nvalid = __popcll(plo) + __popcll(phi);
     }
-    w_exp += (uint32_t)nvalid;
+    if (pact && gl == 0) w_exp += (uint32_t)nvalid;
 
     bool keep[2];
 #pragma unroll
@@ -346,7 +361,7 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
       bool counted = false;
       if (val[h]) {
         const uint32_t x = (uint32_t)(p * B + slot[h]);
-        const int j = h * 64 + lane;
+        const int j = h * 64 + gl;
         if (keep[h]) {
           a.cjob[x] = (uint8_t)j;
           if (last) {
@@ -375,34 +390,23 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
           a.keys[x] = ~0ull;
         }
       }
-      // LDS histogram; one atomic for the whole wave when its digits agree
-      // (the usual case: most children keep their parent's cmax).
-      const u64 cm_mask = __ballot(counted);
-      if (cm_mask) {
-        const int l0 = __ffsll((long long)cm_mask) - 1;
-        const uint32_t d0 = (uint32_t)__shfl((int)d, l0);
-        const u64 same = __ballot(counted && d == d0);
-        if (same == cm_mask) {
-          if (lane == l0) atomicAdd(&s_h[d0], (uint32_t)__popcll(cm_mask));
-        } else if (counted) {
-          atomicAdd(&s_h[d], 1u);
-        }
-      }
+      lds_count(s_h, d, counted, lane);
     }
-    if (!last) {
-      for (int s = lane; s < B; s += 64)
+    if (!last && pact) {
+      for (int s = gl; s < B; s += G)
         if (s >= nvalid) a.keys[(size_t)p * B + s] = ~0ull;
     }
 
-    if (SINGLE && !last) {
+    if (single && !last) {  // G == 64: one parent per wave
+      const int order = (mode == 1) ? kOrderErase : kOrderSwap;
       int pw[kMaxWPL];
 #pragma unroll
       for (int i = 0; i < kMaxWPL; ++i) {
         const int o = lane + 64 * i;
-        pw[i] = (i < NW && o < S) ? par[o] : 0;
+        pw[i] = (pact && i < NW && o < S) ? par[o] : 0;
       }
       int jlast = -1;
-      if (ORDER == kOrderSwap) {
+      if (order == kOrderSwap) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
           const u64 bm = __ballot(rdy[h] && pos[h] == r - 1);
@@ -426,10 +430,18 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
           const int nx1 = __shfl(nxv[h], l) + 1;
           c.newjw = jw_make(nx1, nx1 < m ? r - 1 : kFinishedPos);
           int* out = a.cbuf + ((size_t)p * B + cs) * S;
+          if (order == kOrderErase) {
 #pragma unroll
-          for (int i = 0; i < kMaxWPL; ++i) {
-            const int o = lane + 64 * i;
-            if (i < NW && o < S) out[o] = child_word<ORDER>(o, pw[i], c, Ly);
+            for (int i = 0; i < kMaxWPL; ++i) {
+              const int o = lane + 64 * i;
+              if (i < NW && o < S) out[o] = child_word<kOrderErase>(o, pw[i], c, Ly);
+            }
+          } else {
+#pragma unroll
+            for (int i = 0; i < kMaxWPL; ++i) {
+              const int o = lane + 64 * i;
+              if (i < NW && o < S) out[o] = child_word<kOrderSwap>(o, pw[i], c, Ly);
+            }
           }
         }
       }
@@ -438,18 +450,19 @@ __global__ __launch_bounds__(1024) void k_score(LevelArgs a) {
 
   // Block partials: kept count, expanded count, last-level best; histogram.
   w_cnt = wave_sum(w_cnt);
+  w_exp = wave_sum(w_exp);
   w_best = wave_min64(w_best);
   if (lane == 0) { s_cnt[wave] = w_cnt; s_exp[wave] = w_exp; s_best[wave] = w_best; }
   __syncthreads();
   if (threadIdx.x == 0) {
     BlockStat b;
     b.valid = 0; b.expanded = 0; b.best = ~0ull;
-    for (int w = 0; w < kScoreWaves; ++w) {
+    for (int w = 0; w < nwaves; ++w) {
       b.valid += s_cnt[w];
       b.expanded += s_exp[w];
       b.best = s_best[w] < b.best ? s_best[w] : b.best;
     }
-    a.bstat[blockIdx.x] = b;
+    a.bstat[local] = b;
   }
   if (!last)
     for (int i = threadIdx.x; i < kNB; i += blockDim.x)
@@ -666,7 +679,7 @@ __device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bu
 // Threshold bin from the histogram, block-aggregated compaction of the keys
 // below it (sure survivors) and in it (ties); the last block cuts the ties to
 // exactly the survivors, so cand[0, Kp) holds the Kp survivors.
-__global__ __launch_bounds__(kFilterThreads) void k_filter(LevelArgs a) {
+__device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_ex[kNB];
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_v[16], s_e[16];
@@ -687,7 +700,7 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(LevelArgs a) {
   v = 0; ex = 0;
   for (int w = 0; w < 16; ++w) { v += s_v[w]; ex += s_e[w]; }
   const uint32_t Kp = min(v, (uint32_t)a.K);
-  if (blockIdx.x == 0 && tid == 0) {
+  if (local == 0 && tid == 0) {
     sel->V = v;
     sel->Kp = Kp;
     a.pcount[a.level + 1] = (int)Kp;
@@ -720,13 +733,13 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(LevelArgs a) {
   // smallest child indices, so ties are only marked in a bitmap (by ballot,
   // 64 children per wave = two words) and cut by prefix popcount.
   const bool single = exact ? ((T1 & 7u) != 7u) : (s1 == 0 && T1 < (uint32_t)(kNB - 1));
-  if (blockIdx.x == 0 && tid == 0) {
+  if (local == 0 && tid == 0) {
     sel->T1 = T1; sel->cum_lt = cum_lt; sel->t1 = t1; sel->need = need; sel->single = single ? 1u : 0u;
   }
 
   const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
   const uint32_t W = (N + 31) / 32;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < N; base += gridDim.x * blockDim.x) {
+  for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const u64 key = x < N ? a.keys[x] : ~0ull;
     uint32_t d = kNB;
@@ -765,7 +778,7 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter(LevelArgs a) {
   if (tid == 0) {
     __threadfence();
     const uint32_t ticket = atomicAdd(&sel->done_ctr, 1u);
-    s_last = (ticket == gridDim.x - 1) ? 1 : 0;
+    s_last = (ticket == (uint32_t)nblk - 1) ? 1 : 0;
   }
   __syncthreads();
   if (!s_last) return;
@@ -853,14 +866,13 @@ __device__ __forceinline__ int materialize_one(const LevelArgs& a, const Layout&
 // Rank of every survivor among the Kp survivors (count of smaller keys), then
 // materialise (two-phase) or copy from the child buffer (single-phase) into
 // slot q of the next beam.  One wave per survivor.
-template <int ORDER, bool SINGLE>
-__global__ __launch_bounds__(64 * kRankWaves) void k_rank(LevelArgs a) {
+__device__ void rank_body(const LevelArgs& a, int local, int nblk) {
   extern __shared__ u64 s_c[];
-  __shared__ int s_mc[kRankWaves];
-  __shared__ uint32_t s_mh[kRankWaves], s_ml[kRankWaves];
+  __shared__ int s_mc[16];
+  __shared__ uint32_t s_mh[16], s_ml[16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < kNB; i += gridDim.x * blockDim.x) a.hist_next[i] = 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  for (int i = local * (int)blockDim.x + (int)threadIdx.x; i < kNB; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
+  if (local == 0 && threadIdx.x == 0) {
     a.sel_next->sure_ctr = 0;
     a.sel_next->tie_ctr = 0;
     a.sel_next->done_ctr = 0;
@@ -880,7 +892,10 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank(LevelArgs a) {
   const int S = Ly.S;
   int w_min = INT_MAX;
   uint32_t w_max = 0, w_minh = 0xFFFFFFFFu;
-  for (int u = blockIdx.x * kRankWaves + wave; u < Kp; u += gridDim.x * kRankWaves) {
+  const int nwaves = (int)(blockDim.x >> 6);
+  const bool single = a.cbuf != nullptr;
+  const int order = (a.mode == 1) ? kOrderErase : kOrderSwap;
+  for (int u = local * nwaves + wave; u < Kp; u += nblk * nwaves) {
     const u64 key = u < sf ? C[u] : a.cand[u];
     uint32_t cnt = (uint32_t)u;
     if (u < sf) {
@@ -896,14 +911,15 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank(LevelArgs a) {
     const int p = (int)(x / (uint32_t)a.B);
     const int j = a.cjob[x];
     int cm;
-    if (SINGLE) {
+    if (single) {
       const int* src = a.cbuf + (size_t)x * S;
       int* dst = a.beam_next + (size_t)q * S;
       for (int o = lane; o < S; o += 64) dst[o] = src[o];
       cm = src[Ly.off_cmax];
       if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | j;
     } else {
-      cm = materialize_one<ORDER>(a, Ly, lane, p, j, q);
+      cm = (order == kOrderErase) ? materialize_one<kOrderErase>(a, Ly, lane, p, j, q)
+                                  : materialize_one<kOrderSwap>(a, Ly, lane, p, j, q);
     }
     w_min = min(w_min, cm);
     w_max = max(w_max, (uint32_t)(key >> 32));
@@ -916,12 +932,12 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank(LevelArgs a) {
     rs.min_cmax = INT_MAX;
     rs.max_hi = 0;
     rs.min_hi = 0xFFFFFFFFu;
-    for (int w = 0; w < kRankWaves; ++w) {
+    for (int w = 0; w < nwaves; ++w) {
       rs.min_cmax = min(rs.min_cmax, s_mc[w]);
       rs.max_hi = max(rs.max_hi, s_mh[w]);
       rs.min_hi = min(rs.min_hi, s_ml[w]);
     }
-    a.rstat[blockIdx.x] = rs;
+    a.rstat[local] = rs;
   }
 }
 
@@ -976,7 +992,7 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
 
 // ---------------------------------------------------------------------------
 // Best complete child and its sequence, by walking the trail backwards.
-__global__ __launch_bounds__(256) void k_backtrack(LevelArgs a) {
+__device__ void backtrack_body(const LevelArgs& a) {
   __shared__ int s_cnt[128];
   __shared__ uint32_t s_e[4];
   __shared__ u64 s_b[4];
@@ -1029,6 +1045,106 @@ __global__ __launch_bounds__(256) void k_backtrack(LevelArgs a) {
   }
 }
 
+
+__global__ __launch_bounds__(256) void k_backtrack(LevelArgs a) { backtrack_body(a); }
+
+// ---------------------------------------------------------------------------
+// Step kernels over instance descriptors.
+__device__ __forceinline__ LevelArgs resolve(const InstDesc& d, int level) {
+  LevelArgs a = d.base;
+  a.level = level;
+  a.lseed = d.base.lseed + (uint32_t)level;
+  a.beam_cur = d.beam[level & 1];
+  a.beam_next = d.beam[(level + 1) & 1];
+  a.hist = d.hist[level & 1];
+  a.hist_next = d.hist[(level + 1) & 1];
+  a.sel = d.sel[level & 1];
+  a.sel_prev = d.sel[(level + 1) & 1];
+  a.sel_next = d.sel[(level + 1) & 1];
+  return a;
+}
+
+__global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* map) {
+  InstDesc& d = desc[map[blockIdx.x].x];
+  const LevelArgs& b = d.base;
+  const Layout Ly = make_layout(b.n, b.m);
+  for (int i = threadIdx.x; i < Ly.S; i += blockDim.x) {
+    int v = 0;
+    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, i - Ly.off_jw);  // ready = [0..n-1]
+    d.beam[0][i] = v;
+  }
+  for (int i = threadIdx.x; i < kNB; i += blockDim.x) {
+    d.hist[0][i] = 0;
+    d.hist[1][i] = 0;
+  }
+  if (threadIdx.x == 0) {
+    // Multi-seed passes prune with the shared bound as it stands when the
+    // pass starts, so every beam of a pass sees the same bound.
+    if (d.shared_bound) d.base.best_known = *d.shared_bound;
+    SelState z = {};
+    *d.sel[0] = z;
+    *d.sel[1] = z;
+    b.pcount[0] = 1;
+    *b.expanded = 0ull;
+    b.result[0] = b.best_known; b.result[1] = 0; b.result[2] = 0;
+    d.level = 0;
+    d.bt_done = 0;
+    d.rank_ticket = 0;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_score_b(InstDesc* desc, const BlockRef* map) {
+  const BlockRef r = map[blockIdx.x];
+  const InstDesc& d = desc[r.x];
+  const int level = d.level;
+  if (level >= d.base.L) return;
+  const LevelArgs a = resolve(d, level);
+  score_body(a, r.y, d.nblk_score);
+}
+
+__global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
+  const BlockRef r = map[blockIdx.x];
+  const InstDesc& d = desc[r.x];
+  const int level = d.level;
+  if (level >= d.base.L - 1) return;  // the last level only records the best child
+  const LevelArgs a = resolve(d, level);
+  filter_body(a, r.y, d.nblk_filter);
+}
+
+// The last rank block of an instance to finish advances its level.
+__global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, const BlockRef* map) {
+  const BlockRef r = map[blockIdx.x];
+  InstDesc& d = desc[r.x];
+  const int level = d.level;
+  if (level >= d.base.L) return;
+  if (level < d.base.L - 1) {
+    const LevelArgs a = resolve(d, level);
+    rank_body(a, r.y, d.nblk_rank);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t t = atomicAdd(&d.rank_ticket, 1u);
+    if (t == (uint32_t)d.nblk_rank - 1) {
+      d.rank_ticket = 0;
+      d.level = level + 1;
+      __threadfence();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map) {
+  InstDesc& d = desc[map[blockIdx.x].x];
+  if (d.level < d.base.L || d.bt_done) return;
+  LevelArgs a = resolve(d, d.base.L - 1);
+  a.bt_parent = -1;
+  a.bt_job = -1;
+  backtrack_body(a);
+  if (threadIdx.x == 0) {
+    d.bt_done = 1;
+    if (d.shared_bound && a.result[1]) atomicMin(d.shared_bound, a.result[2]);
+  }
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1037,33 +1153,28 @@ hipError_t launch_init(const LevelArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_score(const LevelArgs& a, int mode, bool single, hipStream_t s) {
-  const dim3 g(a.nblk_score), b(64 * kScoreWaves);
-  if (mode == 1) {
-    if (single) hipLaunchKernelGGL((k_score<1, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_score<1, false>), g, b, 0, s, a);
-  } else {
-    if (single) hipLaunchKernelGGL((k_score<2, true>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((k_score<2, false>), g, b, 0, s, a);
-  }
+hipError_t launch_step_init(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_b, dim3(nblk), dim3(256), 0, s, desc, map);
   return hipGetLastError();
 }
 
-hipError_t launch_filter(const LevelArgs& a, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_filter, dim3(nblk), dim3(kFilterThreads), (size_t)a.bm_words * 4, s, a);
+hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_score_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, desc, map);
   return hipGetLastError();
 }
 
-hipError_t launch_rank(const LevelArgs& a, int order, bool single, hipStream_t s) {
-  const dim3 g(a.nblk_rank), b(64 * kRankWaves);
-  const size_t lds = a.K <= kRankLdsMax ? (size_t)a.K * sizeof(u64) : 0;
-  if (order == kOrderErase) {
-    if (single) hipLaunchKernelGGL((k_rank<kOrderErase, true>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_rank<kOrderErase, false>), g, b, lds, s, a);
-  } else {
-    if (single) hipLaunchKernelGGL((k_rank<kOrderSwap, true>), g, b, lds, s, a);
-    else hipLaunchKernelGGL((k_rank<kOrderSwap, false>), g, b, lds, s, a);
-  }
+hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s) {
+  hipLaunchKernelGGL(k_filter_b, dim3(nblk), dim3(kFilterThreads), lds_bytes, s, desc, map);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s) {
+  hipLaunchKernelGGL(k_rank_b, dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, desc, map);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s) {
+  hipLaunchKernelGGL(k_backtrack_b, dim3(nblk), dim3(256), 0, s, desc, map);
   return hipGetLastError();
 }
 

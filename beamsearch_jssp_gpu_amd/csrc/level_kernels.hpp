@@ -15,9 +15,9 @@ constexpr int kNB = 2048;
 constexpr int kSpreadBins = 10;
 // Shift used at level 0 (no previous level; at most n children anyway).
 constexpr uint32_t kLevel0Shift = 20;
-// Parents per score block (one wave per parent, 1024-thread blocks).
+// Waves per score block (1024 threads; a wave holds 64 / G parents).
 constexpr int kScoreWaves = 16;
-// Waves per filter / rank block.
+// Threads per filter block (one tile of keys per pass) and waves per rank block.
 constexpr int kFilterThreads = 1024;
 constexpr int kRankWaves = 16;
 // Survivors ranked in LDS (8 bytes each) up to this beam width; wider beams
@@ -84,16 +84,44 @@ struct LevelArgs {
   int n, m, S, B, K, L;
   int level, best_known, nblk_score, nblk_rank;
   uint32_t bm_words;         // LDS bitmap words of the filter's threshold cut (dynamic LDS)
-  int exact_digit;           // use the exact (cmax, noise offset) digit (see k_score)
+  int exact_digit;           // use the exact (cmax, noise offset) digit (see score_body)
+  int mode;                  // 1 ICH_DET, 2 PMS_GPU
+  int G;                     // lanes per parent in the score kernel: 16, 32 or 64
   uint32_t lseed;
   int bt_parent, bt_job;     // explicit backtrack start (exact modes), -1 = use bstat best
 };
 
-// mode: 1 ICH_DET, 2 PMS_GPU.  order: kOrderErase / kOrderSwap.
+// Device-resident state of one solve in flight (a batch is an array of them).
+// `base` holds the per-solve constants; the level-dependent views (current /
+// next beam, histogram, selection state, level seed) are resolved in each
+// kernel from the device-side level counter, so one step's launches are
+// identical at every level and can be captured once in a hipGraph.
+struct InstDesc {
+  LevelArgs base;
+  int* beam[2];
+  uint32_t* hist[2];
+  SelState* sel[2];
+  int nblk_score, nblk_filter, nblk_rank;
+  int level;             // next level to run; L when the solve is done
+  int bt_done;           // sequence rebuilt
+  uint32_t rank_ticket;  // last-block ticket of the rank kernel
+  int pad;
+  int* shared_bound;     // multi-seed: device best bound read at init, lowered at the end
+};
+
+// Block map entry: (instance index, block index within that instance).
+typedef int2 BlockRef;
+
+// Step kernels over an array of descriptors; map[b] names the instance and
+// local block of grid block b.
+hipError_t launch_step_init(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
+hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
+hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s);
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s);
+hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
+
+// Exact (host-replay) modes, driven level by level from the host.
 hipError_t launch_init(const LevelArgs& a, hipStream_t s);
-hipError_t launch_score(const LevelArgs& a, int mode, bool single, hipStream_t s);
-hipError_t launch_filter(const LevelArgs& a, int nblk, hipStream_t s);
-hipError_t launch_rank(const LevelArgs& a, int order, bool single, hipStream_t s);
 hipError_t launch_exact_expand(const LevelArgs& a, int P, hipStream_t s);
 hipError_t launch_exact_materialize(const LevelArgs& a, int count, hipStream_t s);
 hipError_t launch_backtrack(const LevelArgs& a, hipStream_t s);

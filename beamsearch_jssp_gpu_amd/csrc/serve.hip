@@ -1,0 +1,444 @@
+// serve.hip -- continuous-batching solve server.
+//
+// Requests are admitted into slots while a slot is free and the HBM budget
+// has room for the request's workspace (first-fit over one device arena of
+// the budget's size, FIFO order).  Every admitted request stays resident on
+// the device until it finishes; each serve step advances all resident
+// requests by a number of levels with the batched step kernels, so work that
+// arrives while others are running overlaps with them on the device.  Results
+// are those of jssp_solve on the same request: a request's beams never depend
+// on what else is resident.  The reference has no serving layer; its drivers
+// solve one instance per process (gpu/v1/main.cu, cpu/v0_ich/main.cpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.hpp"
+#include "level_kernels.hpp"
+#include "workspace.hpp"
+
+namespace {
+
+using jssp::BlockRef;
+using jssp::InstDesc;
+using jssp::Shape;
+using jssp::Work;
+
+enum ReqState { kQueued = 0, kRunning = 1, kDone = 2, kCancelled = 3 };
+
+struct Request {
+  std::vector<int32_t> mach, dur;
+  jssp_instance inst;
+  jssp_params params;
+  double t_submit = 0, t_admit = -1, t_done = -1;
+  int state = kQueued;
+  int slot = -1;
+  int makespan = 0, found = 0;
+  int64_t expanded = 0;
+  std::vector<int32_t> seq;
+};
+
+struct Region {
+  size_t off, size;
+};
+
+}  // namespace
+
+struct jssp_server {
+  jssp_engine* e = nullptr;
+  jssp_server_config cfg{};
+  std::chrono::steady_clock::time_point t0;
+  char* arena = nullptr;  // budget-sized resident-beam arena
+  size_t arena_bytes = 0;
+  std::vector<Region> free_list;  // sorted by offset, coalesced
+  char* meta = nullptr;           // descriptors + block maps
+  InstDesc* d_desc = nullptr;
+  BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_slots = nullptr;
+  size_t cap_map = 0;
+  int nslots = 0;
+  std::vector<uint64_t> slot_ticket;  // 0 = free
+  std::vector<Shape> slot_shape;
+  std::vector<Work> slot_work;
+  std::vector<Region> slot_region;
+  std::deque<uint64_t> queue;
+  std::unordered_map<uint64_t, Request> reqs;
+  uint64_t next_ticket = 1;
+  bool maps_dirty = true;
+  int nms = 0, nmf = 0, nmr = 0, nact = 0;
+  size_t flds = 0, rlds = 0;
+  std::vector<InstDesc> hdesc;
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+namespace {
+
+bool arena_alloc(jssp_server* s, size_t bytes, Region& out) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  for (size_t i = 0; i < s->free_list.size(); ++i) {
+    Region& r = s->free_list[i];
+    if (r.size >= bytes) {
+      out = {r.off, bytes};
+      r.off += bytes;
+      r.size -= bytes;
+      if (r.size == 0) s->free_list.erase(s->free_list.begin() + (long)i);
+      return true;
+    }
+  }
+  return false;
+}
+
+void arena_free(jssp_server* s, Region reg) {
+  auto it = std::lower_bound(s->free_list.begin(), s->free_list.end(), reg,
+                             [](const Region& a, const Region& b) { return a.off < b.off; });
+  it = s->free_list.insert(it, reg);
+  // coalesce with the next and previous blocks
+  if (it + 1 != s->free_list.end() && it->off + it->size == (it + 1)->off) {
+    it->size += (it + 1)->size;
+    s->free_list.erase(it + 1);
+  }
+  if (it != s->free_list.begin() && (it - 1)->off + (it - 1)->size == it->off) {
+    (it - 1)->size += it->size;
+    s->free_list.erase(it);
+  }
+}
+
+// Workspace bytes of a request (two-phase, tuned lanes per parent).
+size_t footprint(jssp_server* s, const Request& r, Shape& shape) {
+  jssp_params q = r.params;
+  q.phases = 2;
+  const int lanes = jssp_engine_tuned_lanes(s->e, r.inst.n_jobs, r.inst.n_machines, q.beam_width);
+  shape = jssp::make_shape(&r.inst, &q, lanes);
+  Work w;
+  return jssp::carve(shape, nullptr, w);
+}
+
+// Admits queued requests (FIFO) while a slot and budget are free.
+int admit(jssp_server* s) {
+  jssp_engine* e = s->e;
+  std::vector<int> fresh;
+  while (!s->queue.empty()) {
+    Request& r = s->reqs[s->queue.front()];
+    if (r.state != kQueued) {
+      s->queue.pop_front();
+      continue;
+    }
+    int slot = -1;
+    for (int i = 0; i < s->nslots; ++i)
+      if (!s->slot_ticket[i]) { slot = i; break; }
+    if (slot < 0) break;
+    Shape shape;
+    const size_t bytes = footprint(s, r, shape);
+    Region reg;
+    if (!arena_alloc(s, bytes, reg)) break;
+    Work w;
+    jssp::carve(shape, s->arena + reg.off, w);
+    jssp_params q = r.params;
+    q.phases = 2;
+    InstDesc d = jssp::make_desc(shape, w, &q, &r.inst);
+    s->hdesc[slot] = d;
+    s->slot_ticket[slot] = s->queue.front();
+    s->slot_shape[slot] = shape;
+    s->slot_work[slot] = w;
+    s->slot_region[slot] = reg;
+    r.slot = slot;
+    r.state = kRunning;
+    r.t_admit = s->now();
+    s->queue.pop_front();
+    fresh.push_back(slot);
+  }
+  if (fresh.empty()) return JSSP_OK;
+  // Upload op tables and descriptors of the admitted requests, then init them.
+  size_t hbytes = fresh.size() * (sizeof(InstDesc) + sizeof(BlockRef));
+  for (int slot : fresh) hbytes += (size_t)s->slot_shape[slot].n * s->slot_shape[slot].m * sizeof(int2);
+  int rc = jssp::ensure_host_staging(e, hbytes + 256);
+  if (rc) return rc;
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  char* h = e->hbuf;
+  size_t off = 0;
+  std::vector<BlockRef> init_map;
+  for (int slot : fresh) {
+    const Request& r = s->reqs[s->slot_ticket[slot]];
+    int2* hops = reinterpret_cast<int2*>(h + off);
+    const int tot = r.inst.n_jobs * r.inst.n_machines;
+    for (int q = 0; q < tot; ++q) { hops[q].x = r.inst.machine[q]; hops[q].y = r.inst.duration[q]; }
+    JSSP_HIP(e, hipMemcpyAsync(s->slot_work[slot].ops, hops, (size_t)tot * sizeof(int2), hipMemcpyHostToDevice,
+                               e->stream));
+    off += (size_t)tot * sizeof(int2);
+    off = (off + 15) & ~(size_t)15;
+    std::memcpy(h + off, &s->hdesc[slot], sizeof(InstDesc));
+    JSSP_HIP(e, hipMemcpyAsync(s->d_desc + slot, h + off, sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
+    off += sizeof(InstDesc);
+    init_map.push_back(make_int2(slot, 0));
+  }
+  std::memcpy(h + off, init_map.data(), init_map.size() * sizeof(BlockRef));
+  JSSP_HIP(e, hipMemcpyAsync(s->d_slots, h + off, init_map.size() * sizeof(BlockRef), hipMemcpyHostToDevice,
+                             e->stream));
+  JSSP_HIP(e, jssp::launch_step_init(s->d_desc, s->d_slots, (int)init_map.size(), e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  s->maps_dirty = true;
+  return JSSP_OK;
+}
+
+int rebuild_maps(jssp_server* s) {
+  jssp_engine* e = s->e;
+  std::vector<BlockRef> ms, mf, mr, ma;
+  s->flds = 0;
+  s->rlds = 0;
+  for (int i = 0; i < s->nslots; ++i) {
+    if (!s->slot_ticket[i]) continue;
+    const Shape& sh = s->slot_shape[i];
+    jssp::append_maps(sh, i, ms, mf, mr);
+    ma.push_back(make_int2(i, 0));
+    const Request& r = s->reqs[s->slot_ticket[i]];
+    s->flds = std::max(s->flds, jssp::filter_lds_bytes(sh, &r.params));
+    s->rlds = std::max(s->rlds, jssp::rank_lds_bytes(sh));
+  }
+  s->nms = (int)ms.size();
+  s->nmf = (int)mf.size();
+  s->nmr = (int)mr.size();
+  s->nact = (int)ma.size();
+  if (ms.size() > s->cap_map || mf.size() > s->cap_map || mr.size() > s->cap_map)
+    return jssp::set_error(e, JSSP_ERR_LIMIT, "serve", "block map capacity exceeded");
+  const size_t hbytes = (ms.size() + mf.size() + mr.size() + ma.size()) * sizeof(BlockRef);
+  int rc = jssp::ensure_host_staging(e, hbytes + 64);
+  if (rc) return rc;
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  char* h = e->hbuf;
+  size_t off = 0;
+  auto put = [&](BlockRef* dst, const std::vector<BlockRef>& v) -> int {
+    if (v.empty()) return JSSP_OK;
+    std::memcpy(h + off, v.data(), v.size() * sizeof(BlockRef));
+    JSSP_HIP(e, hipMemcpyAsync(dst, h + off, v.size() * sizeof(BlockRef), hipMemcpyHostToDevice, e->stream));
+    off += v.size() * sizeof(BlockRef);
+    return JSSP_OK;
+  };
+  if ((rc = put(s->d_ms, ms)) || (rc = put(s->d_mf, mf)) || (rc = put(s->d_mr, mr)) || (rc = put(s->d_slots, ma)))
+    return rc;
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  s->maps_dirty = false;
+  return JSSP_OK;
+}
+
+// Collects finished requests and frees their slots.
+int harvest(jssp_server* s) {
+  jssp_engine* e = s->e;
+  std::vector<InstDesc> ds((size_t)s->nslots);
+  JSSP_HIP(e, hipMemcpyAsync(ds.data(), s->d_desc, (size_t)s->nslots * sizeof(InstDesc), hipMemcpyDeviceToHost,
+                             e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  for (int i = 0; i < s->nslots; ++i) {
+    if (!s->slot_ticket[i] || !ds[i].bt_done) continue;
+    Request& r = s->reqs[s->slot_ticket[i]];
+    const Work& w = s->slot_work[i];
+    int res[4];
+    unsigned long long ex = 0;
+    r.seq.assign((size_t)s->slot_shape[i].L, 0);
+    JSSP_HIP(e, hipMemcpyAsync(res, w.result, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    JSSP_HIP(e, hipMemcpyAsync(&ex, w.expanded, sizeof(ex), hipMemcpyDeviceToHost, e->stream));
+    JSSP_HIP(e, hipMemcpyAsync(r.seq.data(), w.seq, r.seq.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                               e->stream));
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+    r.makespan = res[0];
+    r.found = res[1];
+    r.expanded = (int64_t)ex;
+    r.state = kDone;
+    r.t_done = s->now();
+    arena_free(s, s->slot_region[i]);
+    s->slot_ticket[i] = 0;
+    s->maps_dirty = true;
+  }
+  return JSSP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_server** out) {
+  if (!e || !cfg || !out || cfg->hbm_budget_bytes <= 0 || cfg->max_slots <= 0) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  *out = nullptr;
+  jssp_server* s = new jssp_server();
+  s->e = e;
+  s->cfg = *cfg;
+  s->t0 = std::chrono::steady_clock::now();
+  s->nslots = cfg->max_slots;
+  s->arena_bytes = (size_t)cfg->hbm_budget_bytes;
+  if (hipSetDevice(e->device) != hipSuccess || hipMalloc(&s->arena, s->arena_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    delete s;
+    return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot reserve the HBM budget");
+  }
+  s->free_list.push_back({0, s->arena_bytes});
+  s->cap_map = (size_t)s->nslots * 512;
+  jssp::Bump b;
+  b.take<InstDesc>(s->nslots);
+  for (int k = 0; k < 4; ++k) b.take<BlockRef>(s->cap_map);
+  const size_t meta_bytes = b.off + 256;
+  if (hipMalloc(&s->meta, meta_bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(s->arena);
+    delete s;
+    return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot allocate slot descriptors");
+  }
+  b = jssp::Bump();
+  b.base = s->meta;
+  s->d_desc = b.take<InstDesc>(s->nslots);
+  s->d_ms = b.take<BlockRef>(s->cap_map);
+  s->d_mf = b.take<BlockRef>(s->cap_map);
+  s->d_mr = b.take<BlockRef>(s->cap_map);
+  s->d_slots = b.take<BlockRef>(s->cap_map);
+  s->slot_ticket.assign(s->nslots, 0);
+  s->slot_shape.resize(s->nslots);
+  s->slot_work.resize(s->nslots);
+  s->slot_region.resize(s->nslots);
+  s->hdesc.resize(s->nslots);
+  *out = s;
+  return JSSP_OK;
+}
+
+void jssp_server_destroy(jssp_server* s) {
+  if (!s) return;
+  std::lock_guard<std::recursive_mutex> lk(s->e->mu);
+  (void)hipStreamSynchronize(s->e->stream);
+  if (s->arena) (void)hipFree(s->arena);
+  if (s->meta) (void)hipFree(s->meta);
+  delete s;
+}
+
+int jssp_serve_submit(jssp_server* s, const jssp_instance* inst, const jssp_params* params, uint64_t* ticket) {
+  if (!s || !inst || !params || !ticket) return JSSP_ERR_INVALID;
+  jssp_engine* e = s->e;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  int rc = jssp::check_instance(e, inst);
+  if (rc) return rc;
+  rc = jssp::check_params(e, inst, params);
+  if (rc) return rc;
+  if (params->mode != JSSP_MODE_ICH_DET && params->mode != JSSP_MODE_PMS_GPU)
+    return jssp::set_error(e, JSSP_ERR_INVALID, "serve", "served requests take modes ICH_DET and PMS_GPU");
+  if (params->n_seeds > 1 || params->n_passes > 1)
+    return jssp::set_error(e, JSSP_ERR_INVALID, "serve", "n_seeds / n_passes apply to jssp_solve");
+  Request r;
+  const int tot = inst->n_jobs * inst->n_machines;
+  r.mach.assign(inst->machine, inst->machine + tot);
+  r.dur.assign(inst->duration, inst->duration + tot);
+  r.inst = *inst;
+  r.inst.machine = r.mach.data();
+  r.inst.duration = r.dur.data();
+  r.params = *params;
+  Shape shape;
+  if (footprint(s, r, shape) > s->arena_bytes)
+    return jssp::set_error(e, JSSP_ERR_BUDGET, "serve", "request workspace exceeds the HBM budget");
+  r.t_submit = s->now();
+  const uint64_t t = s->next_ticket++;
+  auto it = s->reqs.emplace(t, std::move(r)).first;
+  it->second.inst.machine = it->second.mach.data();
+  it->second.inst.duration = it->second.dur.data();
+  s->queue.push_back(t);
+  *ticket = t;
+  return JSSP_OK;
+}
+
+int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
+  if (!s || levels < 0) return JSSP_ERR_INVALID;
+  jssp_engine* e = s->e;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  JSSP_HIP(e, hipSetDevice(e->device));
+  int rc = admit(s);
+  if (rc) return rc;
+  if (s->maps_dirty) {
+    rc = rebuild_maps(s);
+    if (rc) return rc;
+  }
+  if (s->nact > 0) {
+    for (int t = 0; t < levels; ++t) {
+      JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, e->stream));
+      JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, e->stream));
+      JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr, s->nmr, s->rlds, e->stream));
+    }
+    JSSP_HIP(e, jssp::launch_step_backtrack(s->d_desc, s->d_slots, s->nact, e->stream));
+    rc = harvest(s);
+    if (rc) return rc;
+  }
+  if (active) {
+    int a = 0;
+    for (int i = 0; i < s->nslots; ++i) a += s->slot_ticket[i] ? 1 : 0;
+    *active = a;
+  }
+  return JSSP_OK;
+}
+
+int jssp_serve_poll(jssp_server* s, uint64_t ticket, jssp_result* res) {
+  if (!s || !res) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(s->e->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return JSSP_ERR_NOTFOUND;
+  const Request& r = it->second;
+  if (r.state == kCancelled) return JSSP_ERR_CANCELLED;
+  if (r.state != kDone) return JSSP_ERR_PENDING;
+  res->makespan = r.makespan;
+  res->found = r.found;
+  res->expanded_states = r.expanded;
+  res->levels = r.inst.n_jobs * r.inst.n_machines;
+  res->best_seed_index = 0;
+  res->device_ms = (r.t_done - r.t_admit) * 1e3;
+  if (r.found && res->sequence) std::memcpy(res->sequence, r.seq.data(), r.seq.size() * sizeof(int32_t));
+  return JSSP_OK;
+}
+
+int jssp_serve_cancel(jssp_server* s, uint64_t ticket) {
+  if (!s) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(s->e->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return JSSP_ERR_NOTFOUND;
+  Request& r = it->second;
+  if (r.state == kDone || r.state == kCancelled) return JSSP_OK;
+  if (r.state == kRunning) {
+    // Every enqueued step has completed (serve_step synchronises), so the
+    // slot can be dropped from the maps and its workspace released.
+    arena_free(s, s->slot_region[r.slot]);
+    s->slot_ticket[r.slot] = 0;
+    s->maps_dirty = true;
+  }
+  r.state = kCancelled;
+  r.t_done = s->now();
+  return JSSP_OK;
+}
+
+int jssp_serve_drain(jssp_server* s) {
+  if (!s) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(s->e->mu);
+  for (;;) {
+    int active = 0;
+    int rc = jssp_serve_step(s, 16, &active);
+    if (rc) return rc;
+    bool queued = false;
+    for (uint64_t t : s->queue)
+      if (s->reqs[t].state == kQueued) { queued = true; break; }
+    if (!active && !queued) return JSSP_OK;
+    if (!active && queued) {
+      // Nothing resident and the head request does not fit: cannot progress.
+      return jssp::set_error(s->e, JSSP_ERR_BUDGET, "serve", "queued request does not fit the HBM budget");
+    }
+  }
+}
+
+int jssp_serve_times(jssp_server* s, uint64_t ticket, double* submit_s, double* admit_s, double* done_s) {
+  if (!s) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(s->e->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return JSSP_ERR_NOTFOUND;
+  if (submit_s) *submit_s = it->second.t_submit;
+  if (admit_s) *admit_s = it->second.t_admit;
+  if (done_s) *done_s = it->second.t_done;
+  return JSSP_OK;
+}
+
+}  // extern "C"

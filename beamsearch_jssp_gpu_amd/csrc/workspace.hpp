@@ -1,0 +1,81 @@
+// workspace.hpp -- device workspace planning and instance descriptors,
+// shared by the single-solve, batch and serving paths.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "engine.hpp"
+#include "level_kernels.hpp"
+
+namespace jssp {
+
+// Device workspace of one solve (one slot of a batch).
+struct Work {
+  int2* ops;
+  int* beam[2];
+  int* cbuf;
+  unsigned long long* keys;
+  uint8_t* cjob;
+  unsigned long long* cand;
+  unsigned long long* ref_buf;
+  uint16_t* kdig;
+  uint32_t* tbits;
+  uint32_t* hist[2];
+  BlockStat* bstat;
+  RankStat* rstat;
+  SelState* sel[2];
+  int* pcount;
+  int* trail;
+  unsigned long long* expanded;
+  uint32_t* lvl_exp;
+  SelState* sel_log;
+  int* seq;
+  int* result;
+  int* ex_cmax;
+  uint8_t* ex_job;
+  uint8_t* ex_mc;
+  int* ex_rcount;
+  int* ex_order;
+};
+
+// Sizes and launch shape of one solve.
+struct Shape {
+  int n, m, S, B, K, L;
+  bool single, exact;
+  int G;                       // lanes per parent in the score kernel
+  int nblk_score, nblk_filter, nblk_rank;
+};
+
+// Lanes per parent: the smallest group that holds every job (64 above 32).
+int default_lanes(int n_jobs);
+// G = 0 picks default_lanes; single-phase always uses 64.
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int G);
+// Bytes of the workspace (base == nullptr) or carve it at base.
+size_t carve(const Shape& s, char* base, Work& w);
+LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p);
+bool exact_digit_applies(const jssp_instance* in);
+// Descriptor of one solve (host copy; upload it to the device array).
+InstDesc make_desc(const Shape& s, const Work& w, const jssp_params* p, const jssp_instance* in);
+// Appends this instance's blocks to the three step maps.
+void append_maps(const Shape& s, int inst, std::vector<BlockRef>& score, std::vector<BlockRef>& filter,
+                 std::vector<BlockRef>& rank);
+// Dynamic LDS of the filter (threshold bitmap) and rank (survivor keys) kernels.
+size_t filter_lds_bytes(const Shape& s, const jssp_params* p);
+size_t rank_lds_bytes(const Shape& s);
+// Writes (machine, duration) pairs to the staging buffer and copies them to d_ops.
+int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops);
+
+// Optional per-kernel HIP-event timing on the engine's stream.
+struct Prof {
+  jssp_engine* e;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
+  size_t next = 0;
+  explicit Prof(jssp_engine* eng) : e(eng) {}
+  hipEvent_t get();
+  hipEvent_t begin();
+  void end(int kind, hipEvent_t a);
+  void collect();
+};
+
+}  // namespace jssp

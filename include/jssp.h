@@ -128,6 +128,19 @@ int jssp_solve_trace(jssp_engine* engine, const jssp_instance* inst, const jssp_
  * jssp_solve on each instance alone.  Modes ICH_DET and PMS_GPU. */
 int jssp_solve_batch(jssp_engine* engine, int32_t count, const jssp_instance* insts,
                      const jssp_params* params, jssp_result* results);
+/* As jssp_solve_batch; use_graph = 0 enqueues the step kernels directly
+ * instead of replaying the captured step graph (same results). */
+int jssp_solve_batch_ex(jssp_engine* engine, int32_t count, const jssp_instance* insts,
+                        const jssp_params* params, jssp_result* results, int32_t use_graph);
+
+/* Lanes-per-parent autotuner of the score kernel (16, 32 or 64 lanes per
+ * parent; tile = 16 waves).  When on (the default), a batch first times a few
+ * levels of each (n, m, K) class it has not seen with every valid shape and
+ * keeps the fastest; table_path (or NULL) persists the table as text lines
+ * "n m log2K lanes".  The shape never changes results. */
+int jssp_engine_set_autotune(jssp_engine* engine, int32_t on, const char* table_path);
+/* Tuned lanes per parent for a class, 0 if not tuned yet. */
+int jssp_engine_tuned_lanes(const jssp_engine* engine, int32_t n_jobs, int32_t n_machines, int32_t beam_width);
 
 /* Per-kernel timing accumulated over the solves run since profiling was
  * switched on (jssp_engine_set_profiling(engine, 1) also resets it), for

@@ -1,0 +1,133 @@
+"""Batched solves, multi-seed search and the serving loop: every result must
+equal the single-instance solve of the same request (bit-exact makespan,
+sequence and expanded count), whatever else shares the device."""
+import numpy as np
+import pytest
+
+import beamsearch_jssp_gpu_amd as bs
+from testlib import orc_gpu, splitmix_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def _b(i):
+    return bs.JSSPInstance(i.machine, i.duration, i.name)
+
+
+def _mixed(seed, count, ns=(10, 15, 20, 30, 50), ms=(5, 10, 15, 20), ks=(32, 64, 128)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for t in range(count):
+        n, m, K = int(rng.choice(ns)), int(rng.choice(ms)), int(rng.choice(ks))
+        mode = int(rng.choice([bs.MODE_ICH_DET, bs.MODE_PMS_GPU]))
+        inst = splitmix_instance(1000 + seed * 100 + t, n, m, 1, 99)
+        out.append((inst, dict(beam_width=K, mode=mode, phases=2, best_known=bs.NO_MAKESPAN, seed=7 + t)))
+    return out
+
+
+def _same(a, b):
+    assert a.makespan == b.makespan and a.found == b.found and a.expanded_states == b.expanded_states
+    if a.found:
+        np.testing.assert_array_equal(a.sequence, b.sequence)
+
+
+def test_batch_equals_single_solves(engine):
+    jobs = _mixed(1, 24)
+    print("batch of", len(jobs), flush=True)
+    singles = [engine.solve(_b(i), **p) for i, p in jobs]
+    for use_graph in (True, False):
+        got = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs], use_graph=use_graph)
+        for g, s in zip(got, singles):
+            _same(g, s)
+    # and against the oracle for a few
+    for (inst, p), g in list(zip(jobs, got))[:4]:
+        w = orc_gpu(inst, p["beam_width"], p["mode"], p["best_known"], p["seed"], trace=False)
+        assert g.makespan == w["makespan"] and g.expanded_states == w["expanded"]
+        np.testing.assert_array_equal(g.sequence, w["sequence"])
+
+
+def test_batch_lane_shapes_do_not_change_results(engine, tmp_path):
+    """Each lanes-per-parent shape (16 / 32 / 64) gives the same beams."""
+    jobs = _mixed(2, 9, ns=(10, 15, 20, 30))
+    want = [engine.solve(_b(i), **p) for i, p in jobs]
+    for lanes in (16, 32, 64):
+        table = tmp_path / f"tune{lanes}.txt"
+        with open(table, "w") as f:
+            for inst, p in jobs:
+                kb = int(np.ceil(np.log2(p["beam_width"])))
+                f.write(f"{inst.n} {inst.m} {kb} {lanes if lanes >= inst.n else 64}\n")
+        eng = bs.Engine(0)
+        eng.set_autotune(True, str(table))
+        got = eng.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+        for g, s in zip(got, want):
+            _same(g, s)
+        for inst, p in jobs:
+            assert eng.tuned_lanes(inst.n, inst.m, p["beam_width"]) in (lanes if lanes >= inst.n else 64,)
+        eng.close()
+
+
+def test_batch_repeated_runs_identical(engine):
+    jobs = _mixed(3, 12)
+    a = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+    engine.solve(_b(jobs[0][0]), **jobs[0][1])  # eager work in between
+    b = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+    c = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+    for x, y, z in zip(a, b, c):
+        _same(x, y)
+        _same(x, z)
+
+
+def test_multiseed_shared_bound(engine):
+    inst = splitmix_instance(11, 15, 10, 1, 99)
+    S, E, K = 6, 2, 64
+    got = engine.solve(_b(inst), K, mode=bs.MODE_PMS_GPU, seed=100, n_seeds=S, n_passes=E)
+    # sequential restatement of the pass structure with the oracle
+    bound, best, best_idx, exp = bs.NO_MAKESPAN, None, -1, 0
+    for e in range(E):
+        pass_bound = bound
+        for i in range(S):
+            r = orc_gpu(inst, K, bs.MODE_PMS_GPU, pass_bound, 100 + e * S + i, trace=False)
+            exp += r["expanded"]
+            if r["found"] and r["makespan"] < bound:
+                bound, best, best_idx = r["makespan"], r["sequence"], e * S + i
+    assert got.makespan == bound
+    assert got.expanded_states == exp
+    assert got.best_seed_index == best_idx
+    np.testing.assert_array_equal(got.sequence, best)
+
+
+def test_serving_alone_equals_in_mix(engine):
+    jobs = _mixed(4, 20, ks=(64, 128, 256))
+    alone = [engine.solve(_b(i), **p) for i, p in jobs]
+    srv = bs.Server(engine, hbm_budget_bytes=256 << 20, max_slots=8)
+    tickets = []
+    for k, (inst, p) in enumerate(jobs):
+        tickets.append(srv.submit(_b(inst), p["beam_width"], p["mode"], p["best_known"], p["seed"]))
+        if k % 3 == 2:
+            srv.step(8)  # requests arrive while others are resident
+    srv.drain()
+    for t, a in zip(tickets, alone):
+        _same(srv.poll(t), a)
+        sub, adm, done = srv.times(t)
+        assert 0 <= sub <= adm <= done
+    srv.close()
+
+
+def test_serving_budget_and_cancel(engine):
+    inst = splitmix_instance(5, 20, 10, 1, 99)
+    srv = bs.Server(engine, hbm_budget_bytes=4 << 20, max_slots=2)
+    with pytest.raises(bs.JsspError) as e:
+        srv.submit(_b(inst), 4096, bs.MODE_ICH_DET)
+    assert e.value.code == 6
+    t1 = srv.submit(_b(inst), 32, bs.MODE_ICH_DET, seed=1)
+    t2 = srv.submit(_b(inst), 32, bs.MODE_ICH_DET, seed=2)
+    t3 = srv.submit(_b(inst), 32, bs.MODE_ICH_DET, seed=3)
+    srv.cancel(t2)
+    srv.drain()
+    assert srv.poll(t1) is not None and srv.poll(t3) is not None
+    with pytest.raises(bs.JsspError) as e:
+        srv.poll(t2)
+    assert e.value.code == 9
+    want = engine.solve(_b(inst), 32, mode=bs.MODE_ICH_DET, seed=3)
+    _same(srv.poll(t3), want)
+    srv.close()
