@@ -182,7 +182,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
 }
 
 int enqueue_step(jssp_engine* e, const BatchPlan& bp) {
-  JSSP_HIP(e, launch_step_score(bp.d_desc, bp.d_ms, bp.nms, e->stream));
+  JSSP_HIP(e, launch_step_score(bp.d_desc, bp.d_ms, bp.nms, false, e->stream));
   JSSP_HIP(e, launch_step_filter(bp.d_desc, bp.d_mf, bp.nmf, bp.flds, e->stream));
   JSSP_HIP(e, launch_step_rank(bp.d_desc, bp.d_mr, bp.nmr, bp.rlds, e->stream));
   return JSSP_OK;

@@ -131,13 +131,14 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.beam[1] = b.take<int>((size_t)s.K * s.S);
   w.cbuf = s.single ? b.take<int>(N * s.S) : nullptr;
   w.keys = b.take<unsigned long long>(std::max(N, (size_t)2 * s.K));
-  w.cjob = b.take<uint8_t>(N);
+  w.crec = b.take<unsigned long long>(N);
   w.cand = b.take<unsigned long long>(N);
   w.ref_buf = b.take<unsigned long long>(N + s.K);
   w.kdig = b.take<uint16_t>(N);
   w.tbits = b.take<uint32_t>((N + 31) / 32 + 2);
-  w.hist[0] = b.take<uint32_t>(kNB);
-  w.hist[1] = b.take<uint32_t>(kNB);
+  w.tcnt = b.take<uint32_t>((N + kFilterThreads - 1) / kFilterThreads + 2);
+  w.hist[0] = b.take<uint32_t>((size_t)kNB * kHistCopies);
+  w.hist[1] = b.take<uint32_t>((size_t)kNB * kHistCopies);
   w.bstat = b.take<BlockStat>(s.nblk_score);
   w.rstat = b.take<RankStat>(s.nblk_rank);
   w.sel[0] = b.take<SelState>(1);
@@ -186,11 +187,12 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.beam_next = w.beam[1];
   a.cbuf = w.cbuf;
   a.keys = w.keys;
-  a.cjob = w.cjob;
+  a.crec = w.crec;
   a.cand = w.cand;
   a.ref_buf = w.ref_buf;
   a.kdig = w.kdig;
   a.tbits = w.tbits;
+  a.tcnt = w.tcnt;
   a.hist = w.hist[0];
   a.hist_next = w.hist[1];
   a.bstat = w.bstat;
@@ -232,7 +234,9 @@ InstDesc make_desc(const Shape& s, const Work& w, const jssp_params* p, const js
   d.nblk_score = s.nblk_score;
   d.nblk_filter = s.nblk_filter;
   d.nblk_rank = s.nblk_rank;
-  d.level = 0;
+  d.lvl_score = 0;
+  d.lvl_filter = 0;
+  d.lvl_rank = -1;
   return d;
 }
 
@@ -248,7 +252,11 @@ size_t filter_lds_bytes(const Shape& s, const jssp_params* p) {
   return 4 * std::min<size_t>(((size_t)s.K * s.B + 31) / 32, kBitmapWordsMax);
 }
 
-size_t rank_lds_bytes(const Shape& s) { return s.K <= kRankLdsMax ? (size_t)s.K * 8 : 0; }
+// Survivor keys (K <= kRankLdsMax) and the prefix of per-tile tie counts.
+size_t rank_lds_bytes(const Shape& s) {
+  const size_t tiles = ((size_t)s.K * s.B + kFilterThreads - 1) / kFilterThreads;
+  return (s.K <= kRankLdsMax ? (size_t)s.K * 8 : 0) + 4 * (tiles + 1);
+}
 
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops) {
   const int total = in->n_jobs * in->n_machines;
@@ -367,7 +375,7 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   JSSP_HIP(e, launch_step_init(d_desc, d_one, 1, e->stream));
   for (int l = 0; l < s.L; ++l) {
     hipEvent_t t0 = prof.begin();
-    JSSP_HIP(e, launch_step_score(d_desc, d_ms, (int)ms.size(), e->stream));
+    JSSP_HIP(e, launch_step_score(d_desc, d_ms, (int)ms.size(), s.single, e->stream));
     prof.end(kKindScore, t0);
     t0 = prof.begin();
     JSSP_HIP(e, launch_step_filter(d_desc, d_mf, (int)mf.size(), flds, e->stream));

@@ -151,6 +151,16 @@ struct ChildMod {
   int mc, f, j, cm, spos, jlast, newjw;
 };
 
+// Record of a kept child, written by the score kernel so that materialising
+// a survivor needs only its parent's words: finish time of the scheduled op,
+// its machine, the job.
+__device__ __forceinline__ u64 pack_rec(int f, int mc, int j) {
+  return (u64)(uint32_t)f | ((u64)(uint32_t)mc << 32) | ((u64)(uint32_t)j << 40);
+}
+__device__ __forceinline__ int rec_finish(u64 r) { return (int)(uint32_t)r; }
+__device__ __forceinline__ int rec_machine(u64 r) { return (int)((r >> 32) & 0xFFu); }
+__device__ __forceinline__ int rec_job(u64 r) { return (int)((r >> 40) & 0xFFu); }
+
 // Word `o` of the child built from parent word `pw`.  Erase order: the jobs
 // behind the removed position move up one (cpu/v0_ich/solver.cpp:29).  Swap
 // order: the last job takes the removed position (gpu/v1/solver.cu:101-106).
@@ -180,6 +190,9 @@ __device__ __forceinline__ int child_word(int o, int pw, const ChildMod& c, cons
 
 // Words per lane of a snapshot (S <= 2*128 + 255 + 1 = 512 -> 8).
 constexpr int kMaxWPL = 8;
+// Filter tiles per rank thread in the tie-count prefix (K*B / kFilterThreads
+// tiles <= 65536 * 128 / 1024 = 8192 = 8 per thread of 1024).
+constexpr int kTileRun = 8;
 
 // ---------------------------------------------------------------------------
 __global__ void k_init(LevelArgs a) {
@@ -190,7 +203,7 @@ __global__ void k_init(LevelArgs a) {
     if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, i - Ly.off_jw);  // ready = [0..n-1]
     root[i] = v;
   }
-  for (int i = threadIdx.x; i < kNB; i += blockDim.x) {
+  for (int i = threadIdx.x; i < kNB * kHistCopies; i += blockDim.x) {
     a.hist[i] = 0;
     a.hist_next[i] = 0;
   }
@@ -210,6 +223,7 @@ __global__ void k_init(LevelArgs a) {
 // holds job j (and j + 64 when n > 64).
 // Mode 1 = ICH_DET (every ready op, slot = ready position),
 // mode 2 = PMS_GPU (ops on the hash-chosen pivot machine, gpu/v1/solver.cu:47-70).
+template <bool SINGLE>
 __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_h[kNB];
   __shared__ uint32_t s_cnt[16], s_exp[16];
@@ -225,7 +239,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   const int ns = a.level + 1;
   const int NW = (S + 63) >> 6;
   const int mode = a.mode;
-  const bool single = a.cbuf != nullptr;
+  constexpr bool single = SINGLE;
 
   // Digit parameters.  Every child's score is >= fl(cmin / ns), cmin = the
   // smallest parent cmax: cmax only grows, correctly rounded division is
@@ -363,7 +377,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
         const uint32_t x = (uint32_t)(p * B + slot[h]);
         const int j = h * 64 + gl;
         if (keep[h]) {
-          a.cjob[x] = (uint8_t)j;
+          a.crec[x] = pack_rec(f[h], mc[h], j);
           if (last) {
             const u64 c = ((u64)(uint32_t)cm[h] << 32) | x;
             w_best = c < w_best ? c : w_best;
@@ -464,9 +478,11 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
     }
     a.bstat[local] = b;
   }
-  if (!last)
+  if (!last) {
+    uint32_t* hc = a.hist + (size_t)(local % kHistCopies) * kNB;
     for (int i = threadIdx.x; i < kNB; i += blockDim.x)
-      if (s_h[i]) atomicAdd(&a.hist[i], s_h[i]);
+      if (s_h[i]) atomicAdd(&hc[i], s_h[i]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -677,8 +693,12 @@ __device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bu
 }
 
 // Threshold bin from the histogram, block-aggregated compaction of the keys
-// below it (sure survivors) and in it (ties); the last block cuts the ties to
-// exactly the survivors, so cand[0, Kp) holds the Kp survivors.
+// below it (sure survivors) into cand[0, cum_lt).  When the threshold bin
+// holds one score value, its survivors are its `need` smallest child indices:
+// its keys are only marked in a bitmap over child indices (by ballot) and
+// counted per tile, and the rank kernel takes the first `need` set bits.
+// Otherwise the bin's keys are compacted too and the last block to finish
+// cuts them, so that cand[0, Kp) holds the Kp survivors.
 __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_ex[kNB];
   __shared__ uint32_t s_w[16];
@@ -690,6 +710,14 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   SelState* sel = a.sel;
 
+  // The histogram loads go out first; they do not depend on the counts.
+  uint32_t h0 = 0, h1 = 0;
+#pragma unroll
+  for (int c = 0; c < kHistCopies; ++c) {
+    const uint2 hv = *reinterpret_cast<const uint2*>(a.hist + (size_t)c * kNB + 2 * tid);
+    h0 += hv.x;
+    h1 += hv.y;
+  }
   uint32_t v = 0, ex = 0;
   for (int i = tid; i < a.nblk_score; i += blockDim.x) { v += a.bstat[i].valid; ex += a.bstat[i].expanded; }
   v = wave_sum(v);
@@ -712,7 +740,6 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   const bool exact = sel->exact != 0;
 
   // Exclusive scan of the histogram: 1024 threads x 2 bins.
-  const uint32_t h0 = a.hist[2 * tid], h1 = a.hist[2 * tid + 1];
   const uint32_t sum = h0 + h1;
   const uint32_t inc = wave_incl_scan(sum, lane);
   if (lane == 63) s_w[wave] = inc;
@@ -727,25 +754,29 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __syncthreads();
   const uint32_t T1 = s_T;
   const uint32_t cum_lt = s_ex[T1];
-  const uint32_t t1 = a.hist[T1];
+  const uint32_t t1 = (T1 + 1 < (uint32_t)kNB ? s_ex[T1 + 1] : v) - cum_lt;
   const uint32_t need = Kp - cum_lt;
-  // A single score value in the threshold bin: its survivors are the `need`
-  // smallest child indices, so ties are only marked in a bitmap (by ballot,
-  // 64 children per wave = two words) and cut by prefix popcount.
+  // A single score value in the threshold bin: exact digit whose offset is
+  // not the clamped one, or generic digit at shift 0 below the clamp bin.
   const bool single = exact ? ((T1 & 7u) != 7u) : (s1 == 0 && T1 < (uint32_t)(kNB - 1));
   if (local == 0 && tid == 0) {
     sel->T1 = T1; sel->cum_lt = cum_lt; sel->t1 = t1; sel->need = need; sel->single = single ? 1u : 0u;
+    sel->sorted_from = single ? cum_lt : Kp;
   }
 
   const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
   const uint32_t W = (N + 31) / 32;
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
-    const u64 key = x < N ? a.keys[x] : ~0ull;
-    uint32_t d = kNB;
-    if (key != ~0ull) d = a.kdig[x];
+    const bool inb = x < N;
+    // Both loads at once: the digit of an invalid slot is never read.
+    const u64 key = inb ? a.keys[x] : ~0ull;
+    const uint32_t dg = inb ? (uint32_t)a.kdig[x] : (uint32_t)kNB;
+    const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;
     const bool tieb = d == T1;
+    const bool tie = tieb && !single;
+    const u64 bs = __ballot(sure), bt = __ballot(tie);
     if (single) {
       const u64 tb = __ballot(tieb);
       const uint32_t w0 = (base + 64u * (uint32_t)wave) >> 5;
@@ -753,10 +784,11 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
         a.tbits[w0] = (uint32_t)tb;
         if (w0 + 1 < W) a.tbits[w0 + 1] = (uint32_t)(tb >> 32);
       }
+      if (lane == 0) s_off[1][wave] = (uint32_t)__popcll(tb);
+    } else if (lane == 0) {
+      s_off[1][wave] = (uint32_t)__popcll(bt);
     }
-    const bool tie = tieb && !single;
-    const u64 bs = __ballot(sure), bt = __ballot(tie);
-    if (lane == 0) { s_off[0][wave] = (uint32_t)__popcll(bs); s_off[1][wave] = (uint32_t)__popcll(bt); }
+    if (lane == 0) s_off[0][wave] = (uint32_t)__popcll(bs);
     __syncthreads();
     if (tid == 0) {
       uint32_t ts = 0, tt = 0;
@@ -766,15 +798,18 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
         ts += cs; tt += ct;
       }
       s_base[0] = ts ? atomicAdd(&sel->sure_ctr, ts) : 0u;
-      s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
+      if (single) a.tcnt[base / blockDim.x] = tt;
+      else s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
     }
     __syncthreads();
     if (sure) a.cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
     if (tie) a.cand[s_base[1] + s_off[1][wave] + lanes_below(bt, lane)] = key;
     __syncthreads();
   }
+  if (single) return;  // the rank kernel takes the ties from the bitmap
 
-  // The last block to finish sees every block's candidates.
+  // The last block to finish sees every block's candidates and cuts the
+  // threshold bin's keys cand[cum_lt, cum_lt + t1) to the `need` survivors.
   if (tid == 0) {
     __threadfence();
     const uint32_t ticket = atomicAdd(&sel->done_ctr, 1u);
@@ -783,41 +818,27 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  if (single) {
-    // Ties in child-index order: the first `need` set bits of the bitmap.
-    const uint32_t wpt = (W + blockDim.x - 1) / blockDim.x;
-    const uint32_t w0 = min(W, tid * wpt), w1 = min(W, w0 + wpt);
-    uint32_t pc = 0;
-    for (uint32_t w = w0; w < w1; ++w) pc += (uint32_t)__popc(a.tbits[w]);
-    const uint32_t pinc = wave_incl_scan(pc, lane);
-    if (lane == 63) s_w[wave] = pinc;
-    __syncthreads();
-    uint32_t r = pinc - pc;
-    for (int w = 0; w < wave; ++w) r += s_w[w];
-    for (uint32_t w = w0; w < w1 && r < need; ++w) {
-      uint32_t bits = a.tbits[w];
-      while (bits && r < need) {
-        const uint32_t x = w * 32 + (uint32_t)(__ffs((int)bits) - 1);
-        bits &= bits - 1;
-        a.cand[cum_lt + r] = a.keys[x];
-        ++r;
-      }
-    }
-    if (tid == 0) sel->sorted_from = cum_lt;
-  } else {
-    if (need < t1) {
-      extern __shared__ uint32_t s_bitmap[];
-      refine_cut(a.cand + cum_lt, t1, need, a.keys, a.ref_buf, a.ref_buf + N, s_bitmap, a.bm_words, N);
-    }
-    if (tid == 0) sel->sorted_from = Kp;
+  if (need < t1) {
+    extern __shared__ uint32_t s_bitmap[];
+    refine_cut(a.cand + cum_lt, t1, need, a.keys, a.ref_buf, a.ref_buf + N, s_bitmap, a.bm_words, N);
   }
 }
 
 // ---------------------------------------------------------------------------
-// Builds survivor q of the next beam from its parent (two-phase) and writes
-// the trail entry.  One wave.  Returns the child's cmax.
+// Word o of a snapshot whose words the wave holds as pw[i] = word lane + 64 i
+// (o wave-uniform).
+__device__ __forceinline__ int wave_word(const int (&pw)[kMaxWPL], int o) {
+  const int i = o >> 6;
+  int v = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxWPL; ++k) v = (k == i) ? pw[k] : v;
+  return __shfl(v, o & 63);
+}
+
+// Builds survivor q of the next beam from its parent p and the child's record
+// (two-phase) and writes the trail entry.  One wave.  Returns the child's cmax.
 template <int ORDER>
-__device__ __forceinline__ int materialize_one(const LevelArgs& a, const Layout& Ly, int lane, int p, int j,
+__device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout& Ly, int lane, int p, u64 rec,
                                                int q) {
   const int S = Ly.S, m = a.m;
   const int NW = (S + 63) >> 6;
@@ -841,15 +862,14 @@ __device__ __forceinline__ int materialize_one(const LevelArgs& a, const Layout&
       if (bm) jlast = 64 * i + (__ffsll((long long)bm) - 1) - Ly.off_jw;
     }
   }
-  const int jwj = par[Ly.off_jw + j];
+  const int j = rec_job(rec);
+  const int jwj = wave_word(pw, Ly.off_jw + j);
   const int nx = jw_next(jwj);
-  const int2 md = a.ops[j * m + nx];
-  const int pred = (m == 1) ? (j > 0 ? par[Ly.off_jr + j - 1] : 0) : par[Ly.off_jr + j];
   ChildMod c;
   c.j = j;
-  c.mc = md.x;
-  c.f = max(par[Ly.off_mr + md.x], pred) + md.y;
-  c.cm = max(par[Ly.off_cmax], c.f);
+  c.mc = rec_machine(rec);
+  c.f = rec_finish(rec);
+  c.cm = max(wave_word(pw, Ly.off_cmax), c.f);
   c.spos = jw_pos(jwj);
   c.jlast = jlast;
   c.newjw = jw_make(nx + 1, nx + 1 < m ? r - 1 : kFinishedPos);
@@ -863,71 +883,152 @@ __device__ __forceinline__ int materialize_one(const LevelArgs& a, const Layout&
   return c.cm;
 }
 
-// Rank of every survivor among the Kp survivors (count of smaller keys), then
-// materialise (two-phase) or copy from the child buffer (single-phase) into
-// slot q of the next beam.  One wave per survivor.
+// Child index of the r-th (0-based) set bit of the threshold-bin bitmap, given
+// the exclusive prefix tp[0..tiles] of the per-tile counts.  One wave.
+__device__ __forceinline__ uint32_t tie_child(const LevelArgs& a, const uint32_t* tp, int tiles, uint32_t r,
+                                              uint32_t W, int lane) {
+  int lo = 0, hi = tiles - 1;  // the last tile c with tp[c] <= r holds it
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tp[mid] <= r) lo = mid;
+    else hi = mid - 1;
+  }
+  const uint32_t rr = r - tp[lo];
+  constexpr int kWordsPerTile = kFilterThreads / 32;
+  const uint32_t wi = (uint32_t)lo * kWordsPerTile + (uint32_t)lane;
+  const uint32_t bits = (lane < kWordsPerTile && wi < W) ? a.tbits[wi] : 0u;
+  const uint32_t pc = (uint32_t)__popc(bits);
+  const uint32_t inc = wave_incl_scan(pc, lane);
+  const bool mine = inc > rr && inc - pc <= rr;
+  uint32_t x = 0;
+  if (mine) {
+    uint32_t b = bits;
+    for (uint32_t k = rr - (inc - pc); k > 0; --k) b &= b - 1;
+    x = wi * 32 + (uint32_t)(__ffs((int)b) - 1);
+  }
+  const u64 bm = __ballot(mine);
+  return (uint32_t)__shfl((int)x, __ffsll((long long)bm) - 1);
+}
+
+// Number of the n keys of C below key, one wave; eight loads in flight per
+// step (C is LDS or global memory, known after inlining).
+__device__ __forceinline__ uint32_t count_below(const u64* C, int n, u64 key, int lane) {
+  uint32_t c = 0;
+  int base = 0;
+  for (; base + 512 <= n; base += 512) {
+    u64 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = C[base + 64 * k + lane];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c += (uint32_t)__popcll(__ballot(v[k] < key));
+  }
+  for (; base < n; base += 64) {
+    const int i = base + lane;
+    c += (uint32_t)__popcll(__ballot(i < n && C[i] < key));
+  }
+  return c;
+}
+
+// Rank of every survivor among the Kp survivors, then materialise it
+// (two-phase) or copy it from the child buffer (single-phase) into slot q of
+// the next beam.  One wave per survivor.  cand[0, sf) is ranked by counting
+// the smaller keys (exact (score bits, child index) order -- the order CUB's
+// stable sort gives the reference); survivors u >= sf are the threshold bin's
+// ties, which sort by child index after every key of cand: survivor u is the
+// (u - sf)-th set bit of the bitmap, at rank u.
 __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
   extern __shared__ u64 s_c[];
   __shared__ int s_mc[16];
-  __shared__ uint32_t s_mh[16], s_ml[16];
+  __shared__ uint32_t s_mh[16], s_ml[16], s_w[16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int i = local * (int)blockDim.x + (int)threadIdx.x; i < kNB; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
-  if (local == 0 && threadIdx.x == 0) {
+  const int tid = threadIdx.x;
+  for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
+  if (local == 0 && tid == 0) {
     a.sel_next->sure_ctr = 0;
     a.sel_next->tie_ctr = 0;
     a.sel_next->done_ctr = 0;
     if (a.sel_log) a.sel_log[a.level] = *a.sel;
   }
   const int Kp = (int)a.sel->Kp;
-  // cand[0, sf) holds keys below every key of cand[sf, Kp), which is already
-  // in order: only the first part needs ranking.
   const int sf = min((int)a.sel->sorted_from, Kp);
   const bool in_lds = a.K <= kRankLdsMax;
-  if (in_lds) {
-    for (int i = threadIdx.x; i < sf; i += blockDim.x) s_c[i] = a.cand[i];
+  const bool ties = sf < Kp;
+  uint32_t* s_tp = reinterpret_cast<uint32_t*>(s_c + (in_lds ? a.K : 0));
+  const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
+  const int tiles = (int)((N + kFilterThreads - 1) / kFilterThreads);
+  // Per-tile tie counts (a run of at most kTileRun per thread) and the
+  // survivor keys are loaded together, then the counts are prefix-summed.
+  const int per = (tiles + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int t0 = min(tiles, tid * per);
+  uint32_t tc[kTileRun];
+#pragma unroll
+  for (int k = 0; k < kTileRun; ++k) tc[k] = (ties && k < per && t0 + k < tiles) ? a.tcnt[t0 + k] : 0u;
+  if (in_lds)
+    for (int i = tid; i < sf; i += blockDim.x) s_c[i] = a.cand[i];
+  if (ties) {
+    uint32_t loc = 0;
+#pragma unroll
+    for (int k = 0; k < kTileRun; ++k) loc += tc[k];
+    const uint32_t inc = wave_incl_scan(loc, lane);
+    if (lane == 63) s_w[wave] = inc;
     __syncthreads();
+    uint32_t run = inc - loc;
+    for (int w = 0; w < wave; ++w) run += s_w[w];
+#pragma unroll
+    for (int k = 0; k < kTileRun; ++k) {
+      if (k < per && t0 + k < tiles) s_tp[t0 + k] = run;
+      run += tc[k];
+    }
+    if (tid == (int)blockDim.x - 1) s_tp[tiles] = run;
   }
-  const u64* C = in_lds ? s_c : a.cand;
+  __syncthreads();
   const Layout Ly = make_layout(a.n, a.m);
   const int S = Ly.S;
+  const uint32_t W = (N + 31) / 32;
   int w_min = INT_MAX;
   uint32_t w_max = 0, w_minh = 0xFFFFFFFFu;
   const int nwaves = (int)(blockDim.x >> 6);
   const bool single = a.cbuf != nullptr;
   const int order = (a.mode == 1) ? kOrderErase : kOrderSwap;
   for (int u = local * nwaves + wave; u < Kp; u += nblk * nwaves) {
-    const u64 key = u < sf ? C[u] : a.cand[u];
-    uint32_t cnt = (uint32_t)u;
+    uint32_t x, cnt, hi;
     if (u < sf) {
-      cnt = 0;
-      for (int base = 0; base < sf; base += 64) {
-        const int v = base + lane;
-        const bool lt = v < sf && C[v] < key;
-        cnt += (uint32_t)__popcll(__ballot(lt));
+      u64 key;
+      if (in_lds) {
+        key = s_c[u];
+        cnt = count_below(s_c, sf, key, lane);
+      } else {
+        key = a.cand[u];
+        cnt = count_below(a.cand, sf, key, lane);
       }
+      x = (uint32_t)key;
+      hi = (uint32_t)(key >> 32);
+    } else {
+      x = tie_child(a, s_tp, tiles, (uint32_t)(u - sf), W, lane);
+      cnt = (uint32_t)u;
+      hi = (uint32_t)(a.keys[x] >> 32);
     }
     const int q = (int)cnt;
-    const uint32_t x = (uint32_t)key;
     const int p = (int)(x / (uint32_t)a.B);
-    const int j = a.cjob[x];
+    const u64 rec = a.crec[x];
     int cm;
     if (single) {
       const int* src = a.cbuf + (size_t)x * S;
       int* dst = a.beam_next + (size_t)q * S;
       for (int o = lane; o < S; o += 64) dst[o] = src[o];
       cm = src[Ly.off_cmax];
-      if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | j;
+      if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | rec_job(rec);
     } else {
-      cm = (order == kOrderErase) ? materialize_one<kOrderErase>(a, Ly, lane, p, j, q)
-                                  : materialize_one<kOrderSwap>(a, Ly, lane, p, j, q);
+      cm = (order == kOrderErase) ? materialize_rec<kOrderErase>(a, Ly, lane, p, rec, q)
+                                  : materialize_rec<kOrderSwap>(a, Ly, lane, p, rec, q);
     }
     w_min = min(w_min, cm);
-    w_max = max(w_max, (uint32_t)(key >> 32));
-    w_minh = min(w_minh, (uint32_t)(key >> 32));
+    w_max = max(w_max, hi);
+    w_minh = min(w_minh, hi);
   }
   if (lane == 0) { s_mc[wave] = w_min; s_mh[wave] = w_max; s_ml[wave] = w_minh; }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     RankStat rs;
     rs.min_cmax = INT_MAX;
     rs.max_hi = 0;
@@ -970,6 +1071,7 @@ __global__ __launch_bounds__(256) void k_exact_expand(LevelArgs a, int P) {
           a.ex_cmax[x] = max(pc, f);
           a.ex_job[x] = (uint8_t)j;
           a.ex_mc[x] = (uint8_t)md.x;
+          a.crec[x] = pack_rec(f, md.x, j);
         }
       }
       r += __popcll(__ballot(rdy));
@@ -985,8 +1087,7 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
   for (int q = blockIdx.x * 4 + wave; q < count; q += gridDim.x * 4) {
     const int x = a.ex_order[q];
     const int p = x / a.B;
-    const int j = a.ex_job[x];
-    (void)materialize_one<kOrderErase>(a, Ly, lane, p, j, q);
+    (void)materialize_rec<kOrderErase>(a, Ly, lane, p, a.crec[x], q);
   }
 }
 
@@ -1022,7 +1123,7 @@ __device__ void backtrack_body(const LevelArgs& a) {
     const uint32_t x = (uint32_t)best;
     const int cmax = (int)(best >> 32);
     p = (int)(x / (uint32_t)a.B);
-    j = a.cjob[x];
+    j = rec_job(a.crec[x]);
     a.result[0] = min(cmax, a.best_known);
     a.result[1] = 1;
     a.result[2] = cmax;
@@ -1073,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
     if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, i - Ly.off_jw);  // ready = [0..n-1]
     d.beam[0][i] = v;
   }
-  for (int i = threadIdx.x; i < kNB; i += blockDim.x) {
+  for (int i = threadIdx.x; i < kNB * kHistCopies; i += blockDim.x) {
     d.hist[0][i] = 0;
     d.hist[1][i] = 0;
   }
@@ -1087,55 +1188,64 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
     b.pcount[0] = 1;
     *b.expanded = 0ull;
     b.result[0] = b.best_known; b.result[1] = 0; b.result[2] = 0;
-    d.level = 0;
+    d.lvl_score = 0;
+    d.lvl_filter = 0;
+    d.lvl_rank = -1;  // the first score launch advances it to 0
     d.bt_done = 0;
-    d.rank_ticket = 0;
   }
 }
 
+// Advances a level counter (block 0 of an instance, after its reader kernel
+// has completed).
+__device__ __forceinline__ void advance(int& counter, int local, int L) {
+  if (local == 0 && threadIdx.x == 0 && counter < L) counter = counter + 1;
+}
+
+// Two-phase score: keys and selection digits only.
 __global__ __launch_bounds__(1024) void k_score_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = map[blockIdx.x];
-  const InstDesc& d = desc[r.x];
-  const int level = d.level;
+  InstDesc& d = desc[r.x];
+  const int level = d.lvl_score;
+  advance(d.lvl_rank, r.y, d.base.L);
   if (level >= d.base.L) return;
   const LevelArgs a = resolve(d, level);
-  score_body(a, r.y, d.nblk_score);
+  score_body<false>(a, r.y, d.nblk_score);
+}
+
+// Single-phase expand: the same plus every kept child's snapshot.
+__global__ __launch_bounds__(1024) void k_expand_b(InstDesc* desc, const BlockRef* map) {
+  const BlockRef r = map[blockIdx.x];
+  InstDesc& d = desc[r.x];
+  const int level = d.lvl_score;
+  advance(d.lvl_rank, r.y, d.base.L);
+  if (level >= d.base.L) return;
+  const LevelArgs a = resolve(d, level);
+  score_body<true>(a, r.y, d.nblk_score);
 }
 
 __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = map[blockIdx.x];
-  const InstDesc& d = desc[r.x];
-  const int level = d.level;
+  InstDesc& d = desc[r.x];
+  const int level = d.lvl_filter;
+  advance(d.lvl_score, r.y, d.base.L);
   if (level >= d.base.L - 1) return;  // the last level only records the best child
   const LevelArgs a = resolve(d, level);
   filter_body(a, r.y, d.nblk_filter);
 }
 
-// The last rank block of an instance to finish advances its level.
 __global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = map[blockIdx.x];
   InstDesc& d = desc[r.x];
-  const int level = d.level;
-  if (level >= d.base.L) return;
-  if (level < d.base.L - 1) {
-    const LevelArgs a = resolve(d, level);
-    rank_body(a, r.y, d.nblk_rank);
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const uint32_t t = atomicAdd(&d.rank_ticket, 1u);
-    if (t == (uint32_t)d.nblk_rank - 1) {
-      d.rank_ticket = 0;
-      d.level = level + 1;
-      __threadfence();
-    }
-  }
+  const int level = d.lvl_rank;
+  advance(d.lvl_filter, r.y, d.base.L);
+  if (level >= d.base.L - 1) return;
+  const LevelArgs a = resolve(d, level);
+  rank_body(a, r.y, d.nblk_rank);
 }
 
 __global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map) {
   InstDesc& d = desc[map[blockIdx.x].x];
-  if (d.level < d.base.L || d.bt_done) return;
+  if (d.lvl_filter < d.base.L || d.bt_done) return;  // lvl_filter reaches L after the last rank
   LevelArgs a = resolve(d, d.base.L - 1);
   a.bt_parent = -1;
   a.bt_job = -1;
@@ -1158,8 +1268,11 @@ hipError_t launch_step_init(InstDesc* desc, const BlockRef* map, int nblk, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_score_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, desc, map);
+hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s) {
+  if (single)
+    hipLaunchKernelGGL(k_expand_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, desc, map);
+  else
+    hipLaunchKernelGGL(k_score_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, desc, map);
   return hipGetLastError();
 }
 

@@ -11,6 +11,10 @@ namespace jssp {
 // form: the score bits above a proven lower bound, shifted so the previous
 // level's survivor spread covers ~2^kSpreadBins bins.
 constexpr int kNB = 2048;
+// The score blocks merge their LDS histograms into kHistCopies copies (block
+// b into copy b % kHistCopies), which the filter sums: same-address atomics
+// serialise, so spreading them cuts the merge time by the copy count.
+constexpr int kHistCopies = 16;
 // The shift makes the previous level's survivor spread cover ~2^10 bins.
 constexpr int kSpreadBins = 10;
 // Shift used at level 0 (no previous level; at most n children anyway).
@@ -46,7 +50,7 @@ struct SelState {
   uint32_t sure_ctr, tie_ctr, done_ctr;  // filter counters, zeroed before the level
   uint32_t exact;           // 1: the digit is the exact (cmax, noise offset) digit
   uint32_t single;          // the threshold bin holds a single score value
-  uint32_t sorted_from;     // cand[sorted_from, Kp) is already in rank order
+  uint32_t sorted_from;     // cand[0, sorted_from) needs ranking; the rest are ties in index order
   uint32_t pad[2];
 };
 
@@ -56,13 +60,14 @@ struct LevelArgs {
   int* beam_next;            // K x S
   int* cbuf;                 // single-phase children, (K*B) x S
   unsigned long long* keys;  // K*B (also the refinement scratch)
-  uint8_t* cjob;             // K*B: job of each child slot
+  unsigned long long* crec;  // K*B: kept child's record, finish | machine << 32 | job << 40
   unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
   unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut
   uint16_t* kdig;            // K*B: first selection digit of each key
   uint32_t* tbits;           // ceil(K*B/32) + 2: bitmap of the keys in a single-valued threshold bin
-  uint32_t* hist;            // kNB, this level
-  uint32_t* hist_next;       // kNB, zeroed for the next level
+  uint32_t* tcnt;            // ceil(K*B/kFilterThreads): keys of that bin per filter tile
+  uint32_t* hist;            // kHistCopies x kNB, this level
+  uint32_t* hist_next;       // kHistCopies x kNB, zeroed for the next level
   BlockStat* bstat;          // nblk_score partials
   RankStat* rstat;           // nblk_rank partials of the previous level's survivors
   SelState* sel;             // this level
@@ -94,7 +99,7 @@ struct LevelArgs {
 // Device-resident state of one solve in flight (a batch is an array of them).
 // `base` holds the per-solve constants; the level-dependent views (current /
 // next beam, histogram, selection state, level seed) are resolved in each
-// kernel from the device-side level counter, so one step's launches are
+// kernel from device-side level counters, so one step's launches are
 // identical at every level and can be captured once in a hipGraph.
 struct InstDesc {
   LevelArgs base;
@@ -102,10 +107,12 @@ struct InstDesc {
   uint32_t* hist[2];
   SelState* sel[2];
   int nblk_score, nblk_filter, nblk_rank;
-  int level;             // next level to run; L when the solve is done
+  // Level each step kernel runs next.  Each counter is advanced by block 0
+  // of the kernel that follows its reader in the step (score -> filter ->
+  // rank -> score), after the reader has completed, so no block ever sees a
+  // counter change while its kernel runs; all saturate at L.
+  int lvl_score, lvl_filter, lvl_rank;
   int bt_done;           // sequence rebuilt
-  uint32_t rank_ticket;  // last-block ticket of the rank kernel
-  int pad;
   int* shared_bound;     // multi-seed: device best bound read at init, lowered at the end
 };
 
@@ -115,7 +122,8 @@ typedef int2 BlockRef;
 // Step kernels over an array of descriptors; map[b] names the instance and
 // local block of grid block b.
 hipError_t launch_step_init(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
-hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
+// single: write every kept child's snapshot (single-phase expand).
+hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s);
 hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s);
 hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s);
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);

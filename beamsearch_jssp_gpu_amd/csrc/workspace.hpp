@@ -16,11 +16,12 @@ struct Work {
   int* beam[2];
   int* cbuf;
   unsigned long long* keys;
-  uint8_t* cjob;
+  unsigned long long* crec;
   unsigned long long* cand;
   unsigned long long* ref_buf;
   uint16_t* kdig;
   uint32_t* tbits;
+  uint32_t* tcnt;
   uint32_t* hist[2];
   BlockStat* bstat;
   RankStat* rstat;
