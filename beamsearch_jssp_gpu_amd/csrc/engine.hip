@@ -135,7 +135,6 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.cand = b.take<unsigned long long>(N);
   w.ref_buf = b.take<unsigned long long>(N + s.K);
   w.kdig = b.take<uint16_t>(N);
-  w.tbits = b.take<uint32_t>((N + 31) / 32 + 2);
   w.tcnt = b.take<uint32_t>((N + kFilterThreads - 1) / kFilterThreads + 2);
   w.hist[0] = b.take<uint32_t>((size_t)kNB * kHistCopies);
   w.hist[1] = b.take<uint32_t>((size_t)kNB * kHistCopies);
@@ -191,7 +190,6 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.cand = w.cand;
   a.ref_buf = w.ref_buf;
   a.kdig = w.kdig;
-  a.tbits = w.tbits;
   a.tcnt = w.tcnt;
   a.hist = w.hist[0];
   a.hist_next = w.hist[1];
@@ -278,18 +276,13 @@ hipEvent_t Prof::get() {
   }
   return e->pev[next++];
 }
-hipEvent_t Prof::begin() {
-  if (!e->profiling) return nullptr;
-  hipEvent_t a = get();
-  if (a) (void)hipEventRecord(a, e->stream);
-  return a;
-}
-void Prof::end(int kind, hipEvent_t a) {
-  if (!e->profiling || !a) return;
-  hipEvent_t b = get();
-  if (!b) return;
-  (void)hipEventRecord(b, e->stream);
-  marks.push_back({kind, {a, b}});
+const EventPair* Prof::pair(int kind, EventPair* ep, bool on) {
+  if (!e->profiling || !on) return nullptr;
+  ep->start = get();
+  ep->stop = get();
+  if (!ep->start || !ep->stop) return nullptr;
+  marks.push_back({kind, {ep->start, ep->stop}});
+  return ep;
 }
 void Prof::collect() {
   for (auto& mk : marks) {
@@ -308,8 +301,8 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
   // Algorithmic bytes per launch: the parent snapshots read (4S each), the
   // children written (4S each, single-phase) and the 8-byte (score, child)
   // keys; the filter reads every key once and writes the survivors;
-  // materialisation reads a parent and writes a child snapshot and the trail
-  // entry, and reads the survivor key.
+  // materialisation reads a parent and the survivor's 8-byte record and
+  // writes a child snapshot.
   const double S4 = 4.0 * s.S;
   for (int l = 0; l < s.L; ++l) {
     const double P = pc[l], C = lexp[l];
@@ -318,7 +311,7 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
     else e->kt[kKindScore].bytes += S4 * P + 8.0 * C;
     if (l == s.L - 1) break;
     e->kt[kKindFilter].bytes += 8.0 * C + 8.0 * Kp;
-    e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 12.0 * Kp;
+    e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 8.0 * Kp;
   }
 }
 
@@ -332,59 +325,36 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   Work w;
   Bump tail;
   const size_t need = carve(s, nullptr, w);
-  std::vector<BlockRef> ms, mf, mr;
-  append_maps(s, 0, ms, mf, mr);
-  const size_t extra = sizeof(InstDesc) + (ms.size() + mf.size() + mr.size() + 1) * sizeof(BlockRef) + 1024;
-  int rc = ensure_device_arena(e, need + extra);
+  int rc = ensure_device_arena(e, need + sizeof(InstDesc) + 1024);
   if (rc) return rc;
   carve(s, e->dbuf, w);
   tail.base = e->dbuf + need;
   InstDesc* d_desc = tail.take<InstDesc>(1);
-  BlockRef* d_ms = tail.take<BlockRef>(ms.size());
-  BlockRef* d_mf = tail.take<BlockRef>(mf.size());
-  BlockRef* d_mr = tail.take<BlockRef>(mr.size());
-  BlockRef* d_one = tail.take<BlockRef>(1);
   rc = upload_ops(e, in, w.ops);
   if (rc) return rc;
   InstDesc desc = make_desc(s, w, &q, in);
   if (e->profiling) desc.base.sel_log = w.sel_log;
-  // Descriptor and maps go up through the pinned staging buffer.
-  const size_t hb = sizeof(InstDesc) + (ms.size() + mf.size() + mr.size() + 1) * sizeof(BlockRef);
-  rc = ensure_host_staging(e, std::max(hb, (size_t)64 + (size_t)s.L * 4 + 64));
+  // The descriptor goes up through the pinned staging buffer; the step
+  // kernels run with no block map (one instance, local block = grid block).
+  rc = ensure_host_staging(e, std::max(sizeof(InstDesc), (size_t)64 + (size_t)s.L * 4 + 64));
   if (rc) return rc;
-  {
-    char* h = e->hbuf;
-    size_t off = 0;
-    auto put = [&](void* dst, const void* src, size_t bytes) -> int {
-      std::memcpy(h + off, src, bytes);
-      JSSP_HIP(e, hipMemcpyAsync(dst, h + off, bytes, hipMemcpyHostToDevice, e->stream));
-      off += bytes;
-      return JSSP_OK;
-    };
-    const BlockRef one = make_int2(0, 0);
-    if ((rc = put(d_desc, &desc, sizeof(InstDesc)))) return rc;
-    if ((rc = put(d_ms, ms.data(), ms.size() * sizeof(BlockRef)))) return rc;
-    if ((rc = put(d_mf, mf.data(), mf.size() * sizeof(BlockRef)))) return rc;
-    if ((rc = put(d_mr, mr.data(), mr.size() * sizeof(BlockRef)))) return rc;
-    if ((rc = put(d_one, &one, sizeof(BlockRef)))) return rc;
-    JSSP_HIP(e, hipStreamSynchronize(e->stream));
-  }
+  std::memcpy(e->hbuf, &desc, sizeof(InstDesc));
+  JSSP_HIP(e, hipMemcpyAsync(d_desc, e->hbuf, sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
   const size_t flds = filter_lds_bytes(s, &q), rlds = rank_lds_bytes(s);
   Prof prof(e);
   JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
-  JSSP_HIP(e, launch_step_init(d_desc, d_one, 1, e->stream));
+  JSSP_HIP(e, launch_step_init(d_desc, nullptr, 1, e->stream));
   for (int l = 0; l < s.L; ++l) {
-    hipEvent_t t0 = prof.begin();
-    JSSP_HIP(e, launch_step_score(d_desc, d_ms, (int)ms.size(), s.single, e->stream));
-    prof.end(kKindScore, t0);
-    t0 = prof.begin();
-    JSSP_HIP(e, launch_step_filter(d_desc, d_mf, (int)mf.size(), flds, e->stream));
-    if (l < s.L - 1) prof.end(kKindFilter, t0);
-    t0 = prof.begin();
-    JSSP_HIP(e, launch_step_rank(d_desc, d_mr, (int)mr.size(), rlds, e->stream));
-    if (l < s.L - 1) prof.end(kKindRank, t0);
+    EventPair ep;
+    JSSP_HIP(e, launch_step_score(d_desc, nullptr, s.nblk_score, s.single, e->stream,
+                                  prof.pair(kKindScore, &ep)));
+    JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, e->stream,
+                                   prof.pair(kKindFilter, &ep, l < s.L - 1)));
+    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, e->stream,
+                                 prof.pair(kKindRank, &ep, l < s.L - 1)));
   }
-  JSSP_HIP(e, launch_step_backtrack(d_desc, d_one, 1, e->stream));
+  JSSP_HIP(e, launch_step_backtrack(d_desc, nullptr, 1, e->stream));
   JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
   int* h_res = reinterpret_cast<int*>(e->hbuf);
   unsigned long long* h_exp = reinterpret_cast<unsigned long long*>(e->hbuf + 32);

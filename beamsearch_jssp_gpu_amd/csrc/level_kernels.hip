@@ -1,5 +1,5 @@
-// level_kernels.hip -- per-level kernels of the single-instance beam step on
-// CDNA4 (gfx950, wave64).
+// level_kernels.hip -- per-level kernels of the beam step on CDNA4 (gfx950,
+// wave64).
 //
 // One level = expand/score -> select top-K -> materialise.  The reference does
 // this with kernel_expand_beam_layer / kernel_score_candidates (one block per
@@ -8,30 +8,34 @@
 // sort of every slot (gpu/v1/solver.cu:255) and a one-thread-per-survivor copy
 // (gpu/v1/solver.cu:160-176, gpu/v2_twophases/solver.cu:150-254).  Here a level
 // is three launches:
-//   k_score   one wave per parent; lane j holds job j, so the ready list, the
-//             pivot filter and the child slots come from ballots/popcounts and
-//             every child's start/finish/cmax is computed in registers.  Each
-//             kept child gets the 64-bit key (score bits << 32 | child index)
-//             and is counted in an LDS histogram of its first selection digit
-//             (kNB bins of the score bits above a proven lower bound), merged
-//             into HBM once per block.  Single-phase also writes the child's
-//             compact snapshot (lanes cover word offsets: coalesced stores).
-//   k_filter  every block scans the histogram (LDS, wave prefix) to find the
-//             bin holding the K-th key, and compacts the keys below it with
+//   k_score   G lanes per parent (G = 16, 32 or 64); lane j holds job j, so the
+//   k_expand  ready list, the pivot filter and the child slots come from
+//             ballots/popcounts and every child's start/finish/cmax is computed
+//             in registers.  Each kept child gets the 64-bit key (score bits <<
+//             32 | child index), a 16-bit first selection digit and an 8-byte
+//             record (finish, machine, job, next machine), and is counted in an
+//             LDS histogram of its digit, merged into one of kHistCopies HBM
+//             copies once per block.  k_expand (single-phase) also writes the
+//             child's compact snapshot (lanes cover word offsets: coalesced).
+//   k_filter  every block sums the histogram copies, finds the bin holding the
+//             K-th key (wave prefix scans) and compacts the keys below it with
 //             block-aggregated offsets.  When that bin holds one score value
-//             (the usual case with the exact digit), its keys are only marked
-//             in a bitmap over child indices, by ballot, and the last block to
-//             finish takes the first `need` set bits -- the ties in index
-//             order.  Otherwise the bin's keys are compacted too and the last
-//             block cuts them by exact score histograms plus the bitmap.
-//   k_rank    each block stages the K' survivor keys in LDS; one wave per
-//             survivor counts the smaller keys (its rank = its slot in the
-//             next beam, i.e. exact (score bits, child index) order -- the
-//             order CUB's stable sort gives the reference) and materialises
-//             the child from its parent (two-phase) or copies it from the
-//             child buffer (single-phase).
-// Every count stays on the device, so the host enqueues a fixed sequence of
-// launches with no readback, and the sequence is graph-capturable.
+//             (the usual case with the exact digit), its survivors are its
+//             smallest child indices: each tile lists its keys of that bin in
+//             index order, with a count, and no block waits for another.
+//             Otherwise the bin's keys are compacted too and the last block to
+//             finish cuts them exactly (score histograms, then index bitmap).
+//   k_rank    each block stages the survivor keys in LDS; one wave per
+//             survivor counts the smaller keys (its rank = its slot in the next
+//             beam, i.e. exact (score bits, child index) order -- the order
+//             CUB's stable sort gives the reference), or takes a tie's rank
+//             from the tile prefix, and materialises the child from its parent
+//             words and record (two-phase) or copies it from the child buffer
+//             (single-phase).
+// Every count stays on the device and every level counter is advanced by the
+// next kernel of the step, so the host enqueues a fixed launch sequence with
+// no readback and no device-scope fence, and the sequence is graph-capturable.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -153,13 +157,20 @@ struct ChildMod {
 
 // Record of a kept child, written by the score kernel so that materialising
 // a survivor needs only its parent's words: finish time of the scheduled op,
-// its machine, the job.
-__device__ __forceinline__ u64 pack_rec(int f, int mc, int j) {
-  return (u64)(uint32_t)f | ((u64)(uint32_t)mc << 32) | ((u64)(uint32_t)j << 40);
+// its machine, the job, the machine of the job's following op.
+__device__ __forceinline__ u64 pack_rec(int f, int mc, int j, int next_mc) {
+  return (u64)(uint32_t)f | ((u64)(uint32_t)mc << 32) | ((u64)(uint32_t)j << 40) |
+         ((u64)(uint32_t)next_mc << 48);
 }
 __device__ __forceinline__ int rec_finish(u64 r) { return (int)(uint32_t)r; }
 __device__ __forceinline__ int rec_machine(u64 r) { return (int)((r >> 32) & 0xFFu); }
 __device__ __forceinline__ int rec_job(u64 r) { return (int)((r >> 40) & 0xFFu); }
+__device__ __forceinline__ int rec_next_machine(u64 r) { return (int)((r >> 48) & 0xFFu); }
+
+// Machine of op k+1 of job j (0 past the last op).
+__device__ __forceinline__ int next_machine(const int2* ops, int j, int m, int k) {
+  return k + 1 < m ? ops[j * m + k + 1].x : 0;
+}
 
 // Word `o` of the child built from parent word `pw`.  Erase order: the jobs
 // behind the removed position move up one (cpu/v0_ich/solver.cpp:29).  Swap
@@ -179,7 +190,7 @@ __device__ __forceinline__ int child_word(int o, int pw, const ChildMod& c, cons
       int ps = jw_pos(v);
       v = (ps != kFinishedPos && ps > c.spos) ? v - 256 : v;
     } else {
-      v = (jj == c.jlast) ? jw_make(jw_next(v), c.spos) : v;
+      v = (jj == c.jlast) ? jw_setpos(v, c.spos) : v;
     }
     v = (jj == c.j) ? c.newjw : v;
   } else {
@@ -193,6 +204,8 @@ constexpr int kMaxWPL = 8;
 // Filter tiles per rank thread in the tie-count prefix (K*B / kFilterThreads
 // tiles <= 65536 * 128 / 1024 = 8192 = 8 per thread of 1024).
 constexpr int kTileRun = 8;
+// Survivor keys per rank thread staged into LDS (kRankLdsMax / 1024).
+constexpr int kCandRun = kRankLdsMax / 1024;
 
 // ---------------------------------------------------------------------------
 __global__ void k_init(LevelArgs a) {
@@ -200,7 +213,8 @@ __global__ void k_init(LevelArgs a) {
   int* root = const_cast<int*>(a.beam_cur);
   for (int i = threadIdx.x; i < Ly.S; i += blockDim.x) {
     int v = 0;
-    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, i - Ly.off_jw);  // ready = [0..n-1]
+    const int j = i - Ly.off_jw;
+    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, j, a.ops[j * a.m].x);  // ready = [0..n-1]
     root[i] = v;
   }
   for (int i = threadIdx.x; i < kNB * kHistCopies; i += blockDim.x) {
@@ -227,6 +241,7 @@ template <bool SINGLE>
 __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_h[kNB];
   __shared__ uint32_t s_cnt[16], s_exp[16];
+  __shared__ int s_kmin[16];
   __shared__ u64 s_best[16];
   __shared__ int s_rmin[16];
   __shared__ uint32_t s_rmax[16], s_rlo[16];
@@ -241,13 +256,17 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   const int mode = a.mode;
   constexpr bool single = SINGLE;
 
-  // Digit parameters.  Every child's score is >= fl(cmin / ns), cmin = the
-  // smallest parent cmax: cmax only grows, correctly rounded division is
-  // monotone, and adding the non-negative noise cannot round below the base.
-  // The generic shift spreads the previous level's survivors' score range
-  // over ~2^kSpreadBins bins.
+  // Digit parameters.  Exact digit: lo = a lower bound of every parent's
+  // cmax, the smallest kept child's cmax of the previous level (written by
+  // that level's filter; 0 for the root).  Generic digit: every child's score
+  // is >= fl(cmin / ns), cmin = the smallest parent cmax: cmax only grows,
+  // correctly rounded division is monotone, and adding the non-negative noise
+  // cannot round below the base; the shift spreads the previous level's
+  // survivors' score range over ~2^kSpreadBins bins.
   uint32_t lo = 0, s1 = kLevel0Shift;
-  if (a.level > 0 && !last) {
+  if (a.exact_digit) {
+    lo = a.sel->lo;
+  } else if (a.level > 0 && !last) {
     int cmin = INT_MAX;
     uint32_t mxh = 0, mnh = 0xFFFFFFFFu;
     for (int i = threadIdx.x; i < a.nblk_rank; i += blockDim.x) {
@@ -268,18 +287,16 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
       mxh = max(mxh, s_rmax[w]);
       mnh = min(mnh, s_rlo[w]);
     }
-    if (a.exact_digit) {
-      lo = cmin == INT_MAX ? 0u : (uint32_t)cmin;
-    } else {
-      if (cmin != INT_MAX) lo = __float_as_uint((float)cmin / (float)ns);
-      const uint32_t spread = mxh > mnh ? mxh - mnh : 0u;
-      const int bl = spread ? 32 - __clz(spread) : 0;
-      s1 = (uint32_t)max(0, bl - kSpreadBins);
-    }
-  } else if (a.exact_digit) {
-    lo = 0;  // the root's cmax
+    if (cmin != INT_MAX) lo = __float_as_uint((float)cmin / (float)ns);
+    const uint32_t spread = mxh > mnh ? mxh - mnh : 0u;
+    const int bl = spread ? 32 - __clz(spread) : 0;
+    s1 = (uint32_t)max(0, bl - kSpreadBins);
   }
-  if (local == 0 && threadIdx.x == 0) { a.sel->lo = lo; a.sel->s1 = s1; a.sel->exact = a.exact_digit ? 1u : 0u; }
+  if (local == 0 && threadIdx.x == 0) {
+    if (!a.exact_digit) a.sel->lo = lo;
+    a.sel->s1 = s1;
+    a.sel->exact = a.exact_digit ? 1u : 0u;
+  }
   for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
   __syncthreads();
 
@@ -291,6 +308,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   const int NH = (G == 64) ? ((n + 63) >> 6) : 1;
 
   uint32_t w_cnt = 0, w_exp = 0;
+  int w_kmin = INT_MAX;
   u64 w_best = ~0ull;
 
   for (int pb = (local * nwaves + wave) * PPW; pb < P; pb += nblk * nwaves * PPW) {
@@ -298,26 +316,29 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
     const bool pact = p < P;
     const int* par = a.beam_cur + (size_t)(pact ? p : 0) * S;
     const int pc = pact ? par[Ly.off_cmax] : 0;
-    int pos[2], mc[2], f[2], cm[2], nxv[2];
+    int pos[2], mc[2], f[2], cm[2], nxv[2], nmc[2];
     bool rdy[2];
     int r = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      rdy[h] = false; pos[h] = kFinishedPos; mc[h] = 0; f[h] = 0; cm[h] = 0; nxv[h] = 0;
+      rdy[h] = false; pos[h] = kFinishedPos; mc[h] = 0; f[h] = 0; cm[h] = 0; nxv[h] = 0; nmc[h] = 0;
       const int j = h * 64 + gl;
       if (pact && h < NH && j < n) {
         const int w = par[Ly.off_jw + j];
+        // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
+        const int pred = (m == 1) ? (j > 0 ? par[Ly.off_jr + j - 1] : 0) : par[Ly.off_jr + j];
         pos[h] = jw_pos(w);
         if (pos[h] != kFinishedPos) {
           rdy[h] = true;
           const int nx = jw_next(w);
           nxv[h] = nx;
-          const int2 md = a.ops[j * m + nx];
-          // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
-          const int pred = (m == 1) ? (j > 0 ? par[Ly.off_jr + j - 1] : 0) : par[Ly.off_jr + j];
-          const int st = max(par[Ly.off_mr + md.x], pred);
-          mc[h] = md.x;
-          f[h] = st + md.y;
+          mc[h] = jw_mc(w);
+          // Duration, the following op's machine and the machine's ready
+          // time all depend on the job word only: one round of loads.
+          const int dur = a.ops[j * m + nx].y;
+          nmc[h] = next_machine(a.ops, j, m, nx);
+          const int st = max(par[Ly.off_mr + mc[h]], pred);
+          f[h] = st + dur;
           cm[h] = max(pc, f[h]);
         }
       }
@@ -370,6 +391,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
     bool keep[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
+      val[h] = val[h] && slot[h] < B;  // always true for a consistent snapshot; keeps writes in bounds
       keep[h] = val[h] && cm[h] < a.best_known;
       uint32_t d = 0;
       bool counted = false;
@@ -377,7 +399,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
         const uint32_t x = (uint32_t)(p * B + slot[h]);
         const int j = h * 64 + gl;
         if (keep[h]) {
-          a.crec[x] = pack_rec(f[h], mc[h], j);
+          a.crec[x] = pack_rec(f[h], mc[h], j, nmc[h]);
           if (last) {
             const u64 c = ((u64)(uint32_t)cm[h] << 32) | x;
             w_best = c < w_best ? c : w_best;
@@ -399,6 +421,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
             a.kdig[x] = (uint16_t)d;
             counted = true;
             w_cnt++;
+            w_kmin = min(w_kmin, cm[h]);
           }
         } else if (!last) {
           a.keys[x] = ~0ull;
@@ -442,7 +465,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
           c.jlast = jlast;
           const int cs = __shfl(slot[h], l);
           const int nx1 = __shfl(nxv[h], l) + 1;
-          c.newjw = jw_make(nx1, nx1 < m ? r - 1 : kFinishedPos);
+          c.newjw = nx1 < m ? jw_make(nx1, r - 1, __shfl(nmc[h], l)) : jw_make(nx1, kFinishedPos, 0);
           int* out = a.cbuf + ((size_t)p * B + cs) * S;
           if (order == kOrderErase) {
 #pragma unroll
@@ -465,15 +488,17 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   // Block partials: kept count, expanded count, last-level best; histogram.
   w_cnt = wave_sum(w_cnt);
   w_exp = wave_sum(w_exp);
+  w_kmin = wave_min_i(w_kmin);
   w_best = wave_min64(w_best);
-  if (lane == 0) { s_cnt[wave] = w_cnt; s_exp[wave] = w_exp; s_best[wave] = w_best; }
+  if (lane == 0) { s_cnt[wave] = w_cnt; s_exp[wave] = w_exp; s_kmin[wave] = w_kmin; s_best[wave] = w_best; }
   __syncthreads();
   if (threadIdx.x == 0) {
     BlockStat b;
-    b.valid = 0; b.expanded = 0; b.best = ~0ull;
+    b.valid = 0; b.expanded = 0; b.min_cmax = INT_MAX; b.pad = 0; b.best = ~0ull;
     for (int w = 0; w < nwaves; ++w) {
       b.valid += s_cnt[w];
       b.expanded += s_exp[w];
+      b.min_cmax = min(b.min_cmax, s_kmin[w]);
       b.best = s_best[w] < b.best ? s_best[w] : b.best;
     }
     a.bstat[local] = b;
@@ -695,14 +720,16 @@ __device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bu
 // Threshold bin from the histogram, block-aggregated compaction of the keys
 // below it (sure survivors) into cand[0, cum_lt).  When the threshold bin
 // holds one score value, its survivors are its `need` smallest child indices:
-// its keys are only marked in a bitmap over child indices (by ballot) and
-// counted per tile, and the rank kernel takes the first `need` set bits.
+// each tile (one block pass of kFilterThreads children) writes its keys of
+// that bin in index order at the tile's offset in ref_buf and their count in
+// tcnt, and the rank kernel takes the first `need` of them tile by tile.
 // Otherwise the bin's keys are compacted too and the last block to finish
 // cuts them, so that cand[0, Kp) holds the Kp survivors.
 __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_ex[kNB];
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_v[16], s_e[16];
+  __shared__ int s_km[16];
   __shared__ uint32_t s_T;
   __shared__ uint32_t s_off[2][16];
   __shared__ uint32_t s_base[2];
@@ -719,16 +746,24 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     h1 += hv.y;
   }
   uint32_t v = 0, ex = 0;
-  for (int i = tid; i < a.nblk_score; i += blockDim.x) { v += a.bstat[i].valid; ex += a.bstat[i].expanded; }
+  int km = INT_MAX;
+  for (int i = tid; i < a.nblk_score; i += blockDim.x) {
+    v += a.bstat[i].valid;
+    ex += a.bstat[i].expanded;
+    km = min(km, a.bstat[i].min_cmax);
+  }
   v = wave_sum(v);
   ex = wave_sum(ex);
-  if (lane == 0) { s_v[wave] = v; s_e[wave] = ex; }
+  km = wave_min_i(km);
+  if (lane == 0) { s_v[wave] = v; s_e[wave] = ex; s_km[wave] = km; }
   if (tid == 0) s_T = kNB - 1;
   __syncthreads();
-  v = 0; ex = 0;
-  for (int w = 0; w < 16; ++w) { v += s_v[w]; ex += s_e[w]; }
+  v = 0; ex = 0; km = INT_MAX;
+  for (int w = 0; w < 16; ++w) { v += s_v[w]; ex += s_e[w]; km = min(km, s_km[w]); }
   const uint32_t Kp = min(v, (uint32_t)a.K);
   if (local == 0 && tid == 0) {
+    // Lower bound of the next level's parents' cmax (they are kept children).
+    if (a.exact_digit && km != INT_MAX) a.sel_next->lo = (uint32_t)km;
     sel->V = v;
     sel->Kp = Kp;
     a.pcount[a.level + 1] = (int)Kp;
@@ -737,7 +772,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   }
   if (Kp == 0) return;
   const uint32_t s1 = sel->s1;
-  const bool exact = sel->exact != 0;
+  const bool exact = a.exact_digit != 0;
 
   // Exclusive scan of the histogram: 1024 threads x 2 bins.
   const uint32_t sum = h0 + h1;
@@ -765,7 +800,6 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   }
 
   const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
-  const uint32_t W = (N + 31) / 32;
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
@@ -774,21 +808,12 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     const uint32_t dg = inb ? (uint32_t)a.kdig[x] : (uint32_t)kNB;
     const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;
-    const bool tieb = d == T1;
-    const bool tie = tieb && !single;
+    const bool tie = d == T1;
     const u64 bs = __ballot(sure), bt = __ballot(tie);
-    if (single) {
-      const u64 tb = __ballot(tieb);
-      const uint32_t w0 = (base + 64u * (uint32_t)wave) >> 5;
-      if (lane == 0 && w0 < W) {
-        a.tbits[w0] = (uint32_t)tb;
-        if (w0 + 1 < W) a.tbits[w0 + 1] = (uint32_t)(tb >> 32);
-      }
-      if (lane == 0) s_off[1][wave] = (uint32_t)__popcll(tb);
-    } else if (lane == 0) {
+    if (lane == 0) {
+      s_off[0][wave] = (uint32_t)__popcll(bs);
       s_off[1][wave] = (uint32_t)__popcll(bt);
     }
-    if (lane == 0) s_off[0][wave] = (uint32_t)__popcll(bs);
     __syncthreads();
     if (tid == 0) {
       uint32_t ts = 0, tt = 0;
@@ -803,10 +828,16 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     }
     __syncthreads();
     if (sure) a.cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
-    if (tie) a.cand[s_base[1] + s_off[1][wave] + lanes_below(bt, lane)] = key;
+    if (tie) {
+      // Single-valued bin: the tile's ties in child-index order at the
+      // tile's own offset (no atomics); otherwise compacted after the sure keys.
+      const uint32_t o = s_off[1][wave] + lanes_below(bt, lane);
+      if (single) a.ref_buf[base + o] = key;
+      else a.cand[s_base[1] + o] = key;
+    }
     __syncthreads();
   }
-  if (single) return;  // the rank kernel takes the ties from the bitmap
+  if (single) return;  // the rank kernel takes the ties tile by tile
 
   // The last block to finish sees every block's candidates and cuts the
   // threshold bin's keys cand[cum_lt, cum_lt + t1) to the `need` survivors.
@@ -872,7 +903,7 @@ __device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout&
   c.cm = max(wave_word(pw, Ly.off_cmax), c.f);
   c.spos = jw_pos(jwj);
   c.jlast = jlast;
-  c.newjw = jw_make(nx + 1, nx + 1 < m ? r - 1 : kFinishedPos);
+  c.newjw = nx + 1 < m ? jw_make(nx + 1, r - 1, rec_next_machine(rec)) : jw_make(nx + 1, kFinishedPos, 0);
   int* out = a.beam_next + (size_t)q * S;
 #pragma unroll
   for (int i = 0; i < kMaxWPL; ++i) {
@@ -883,31 +914,17 @@ __device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout&
   return c.cm;
 }
 
-// Child index of the r-th (0-based) set bit of the threshold-bin bitmap, given
-// the exclusive prefix tp[0..tiles] of the per-tile counts.  One wave.
-__device__ __forceinline__ uint32_t tie_child(const LevelArgs& a, const uint32_t* tp, int tiles, uint32_t r,
-                                              uint32_t W, int lane) {
+// Key of the r-th (0-based) tie in child-index order: tile c holds tcnt[c]
+// ties at ref_buf[c * kFilterThreads ...], tp[0..tiles] is the exclusive
+// prefix of those counts.  Wave-uniform.
+__device__ __forceinline__ u64 tie_key(const LevelArgs& a, const uint32_t* tp, int tiles, uint32_t r) {
   int lo = 0, hi = tiles - 1;  // the last tile c with tp[c] <= r holds it
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (tp[mid] <= r) lo = mid;
     else hi = mid - 1;
   }
-  const uint32_t rr = r - tp[lo];
-  constexpr int kWordsPerTile = kFilterThreads / 32;
-  const uint32_t wi = (uint32_t)lo * kWordsPerTile + (uint32_t)lane;
-  const uint32_t bits = (lane < kWordsPerTile && wi < W) ? a.tbits[wi] : 0u;
-  const uint32_t pc = (uint32_t)__popc(bits);
-  const uint32_t inc = wave_incl_scan(pc, lane);
-  const bool mine = inc > rr && inc - pc <= rr;
-  uint32_t x = 0;
-  if (mine) {
-    uint32_t b = bits;
-    for (uint32_t k = rr - (inc - pc); k > 0; --k) b &= b - 1;
-    x = wi * 32 + (uint32_t)(__ffs((int)b) - 1);
-  }
-  const u64 bm = __ballot(mine);
-  return (uint32_t)__shfl((int)x, __ffsll((long long)bm) - 1);
+  return a.ref_buf[(size_t)lo * kFilterThreads + (r - tp[lo])];
 }
 
 // Number of the n keys of C below key, one wave; eight loads in flight per
@@ -931,15 +948,15 @@ __device__ __forceinline__ uint32_t count_below(const u64* C, int n, u64 key, in
 
 // Rank of every survivor among the Kp survivors, then materialise it
 // (two-phase) or copy it from the child buffer (single-phase) into slot q of
-// the next beam.  One wave per survivor.  cand[0, sf) is ranked by counting
-// the smaller keys (exact (score bits, child index) order -- the order CUB's
-// stable sort gives the reference); survivors u >= sf are the threshold bin's
-// ties, which sort by child index after every key of cand: survivor u is the
-// (u - sf)-th set bit of the bitmap, at rank u.
+// the next beam.  One wave per survivor.  cand[0, sf) is staged in LDS and
+// each survivor's rank is the count of smaller keys (exact (score bits, child
+// index) order -- the order CUB's stable sort gives the reference).
+// Survivors u >= sf are the threshold bin's ties, which sort by child index
+// after every key of cand: survivor u is the (u - sf)-th tie, at rank u.
 __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
   extern __shared__ u64 s_c[];
-  __shared__ int s_mc[16];
-  __shared__ uint32_t s_mh[16], s_ml[16], s_w[16];
+  __shared__ int s_mc[kRankWaves];
+  __shared__ uint32_t s_mh[kRankWaves], s_ml[kRankWaves], s_w[kRankWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
   for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
@@ -949,22 +966,40 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
     a.sel_next->done_ctr = 0;
     if (a.sel_log) a.sel_log[a.level] = *a.sel;
   }
+  // Everything this block stages is loaded before the selection state
+  // arrives: the first K candidate keys (cand holds K*B) and the per-tile
+  // tie counts of the widest level (runs of kTileRun per thread).
+  const bool in_lds = a.K <= kRankLdsMax;
+  const int tiles_max = (int)(((uint32_t)a.K * (uint32_t)a.B + kFilterThreads - 1) / kFilterThreads);
+  const int per = (tiles_max + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int t0 = min(tiles_max, tid * per);
+  u64 cv[kCandRun];
+  uint32_t tc[kTileRun];
+  if (in_lds) {
+#pragma unroll
+    for (int k = 0; k < kCandRun; ++k) {
+      const int i = tid + k * (int)blockDim.x;
+      cv[k] = i < a.K ? a.cand[i] : ~0ull;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kTileRun; ++k) tc[k] = (k < per && t0 + k < tiles_max) ? a.tcnt[t0 + k] : 0u;
   const int Kp = (int)a.sel->Kp;
   const int sf = min((int)a.sel->sorted_from, Kp);
-  const bool in_lds = a.K <= kRankLdsMax;
   const bool ties = sf < Kp;
   uint32_t* s_tp = reinterpret_cast<uint32_t*>(s_c + (in_lds ? a.K : 0));
   const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
   const int tiles = (int)((N + kFilterThreads - 1) / kFilterThreads);
-  // Per-tile tie counts (a run of at most kTileRun per thread) and the
-  // survivor keys are loaded together, then the counts are prefix-summed.
-  const int per = (tiles + (int)blockDim.x - 1) / (int)blockDim.x;
-  const int t0 = min(tiles, tid * per);
-  uint32_t tc[kTileRun];
+  if (in_lds) {
 #pragma unroll
-  for (int k = 0; k < kTileRun; ++k) tc[k] = (ties && k < per && t0 + k < tiles) ? a.tcnt[t0 + k] : 0u;
-  if (in_lds)
-    for (int i = tid; i < sf; i += blockDim.x) s_c[i] = a.cand[i];
+    for (int k = 0; k < kCandRun; ++k) {
+      const int i = tid + k * (int)blockDim.x;
+      if (i < sf) s_c[i] = cv[k];
+    }
+  }
+  // Tiles past this level's count hold stale counts.
+#pragma unroll
+  for (int k = 0; k < kTileRun; ++k) tc[k] = (t0 + k < tiles) ? tc[k] : 0u;
   if (ties) {
     uint32_t loc = 0;
 #pragma unroll
@@ -984,16 +1019,15 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
   __syncthreads();
   const Layout Ly = make_layout(a.n, a.m);
   const int S = Ly.S;
-  const uint32_t W = (N + 31) / 32;
   int w_min = INT_MAX;
   uint32_t w_max = 0, w_minh = 0xFFFFFFFFu;
-  const int nwaves = (int)(blockDim.x >> 6);
+  const int nwaves = kRankWaves;
   const bool single = a.cbuf != nullptr;
   const int order = (a.mode == 1) ? kOrderErase : kOrderSwap;
   for (int u = local * nwaves + wave; u < Kp; u += nblk * nwaves) {
-    uint32_t x, cnt, hi;
+    uint32_t cnt = (uint32_t)u;
+    u64 key;
     if (u < sf) {
-      u64 key;
       if (in_lds) {
         key = s_c[u];
         cnt = count_below(s_c, sf, key, lane);
@@ -1001,14 +1035,12 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
         key = a.cand[u];
         cnt = count_below(a.cand, sf, key, lane);
       }
-      x = (uint32_t)key;
-      hi = (uint32_t)(key >> 32);
     } else {
-      x = tie_child(a, s_tp, tiles, (uint32_t)(u - sf), W, lane);
-      cnt = (uint32_t)u;
-      hi = (uint32_t)(a.keys[x] >> 32);
+      key = tie_key(a, s_tp, tiles, (uint32_t)(u - sf));
     }
     const int q = (int)cnt;
+    const uint32_t x = (uint32_t)key;
+    const uint32_t hi = (uint32_t)(key >> 32);
     const int p = (int)(x / (uint32_t)a.B);
     const u64 rec = a.crec[x];
     int cm;
@@ -1071,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_exact_expand(LevelArgs a, int P) {
           a.ex_cmax[x] = max(pc, f);
           a.ex_job[x] = (uint8_t)j;
           a.ex_mc[x] = (uint8_t)md.x;
-          a.crec[x] = pack_rec(f, md.x, j);
+          a.crec[x] = pack_rec(f, md.x, j, next_machine(a.ops, j, m, nx));
         }
       }
       r += __popcll(__ballot(rdy));
@@ -1151,6 +1183,12 @@ __global__ __launch_bounds__(256) void k_backtrack(LevelArgs a) { backtrack_body
 
 // ---------------------------------------------------------------------------
 // Step kernels over instance descriptors.
+// (instance, local block) of this block; a null map means one instance whose
+// local block is the grid block (saves the map load on single solves).
+__device__ __forceinline__ BlockRef block_ref(const BlockRef* map) {
+  return map ? map[blockIdx.x] : make_int2(0, (int)blockIdx.x);
+}
+
 __device__ __forceinline__ LevelArgs resolve(const InstDesc& d, int level) {
   LevelArgs a = d.base;
   a.level = level;
@@ -1166,12 +1204,13 @@ __device__ __forceinline__ LevelArgs resolve(const InstDesc& d, int level) {
 }
 
 __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* map) {
-  InstDesc& d = desc[map[blockIdx.x].x];
+  InstDesc& d = desc[block_ref(map).x];
   const LevelArgs& b = d.base;
   const Layout Ly = make_layout(b.n, b.m);
   for (int i = threadIdx.x; i < Ly.S; i += blockDim.x) {
     int v = 0;
-    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, i - Ly.off_jw);  // ready = [0..n-1]
+    const int j = i - Ly.off_jw;
+    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, j, b.ops[j * b.m].x);  // ready = [0..n-1]
     d.beam[0][i] = v;
   }
   for (int i = threadIdx.x; i < kNB * kHistCopies; i += blockDim.x) {
@@ -1203,7 +1242,7 @@ __device__ __forceinline__ void advance(int& counter, int local, int L) {
 
 // Two-phase score: keys and selection digits only.
 __global__ __launch_bounds__(1024) void k_score_b(InstDesc* desc, const BlockRef* map) {
-  const BlockRef r = map[blockIdx.x];
+  const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_score;
   advance(d.lvl_rank, r.y, d.base.L);
@@ -1214,7 +1253,7 @@ __global__ __launch_bounds__(1024) void k_score_b(InstDesc* desc, const BlockRef
 
 // Single-phase expand: the same plus every kept child's snapshot.
 __global__ __launch_bounds__(1024) void k_expand_b(InstDesc* desc, const BlockRef* map) {
-  const BlockRef r = map[blockIdx.x];
+  const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_score;
   advance(d.lvl_rank, r.y, d.base.L);
@@ -1224,7 +1263,7 @@ __global__ __launch_bounds__(1024) void k_expand_b(InstDesc* desc, const BlockRe
 }
 
 __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
-  const BlockRef r = map[blockIdx.x];
+  const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_filter;
   advance(d.lvl_score, r.y, d.base.L);
@@ -1234,7 +1273,7 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, con
 }
 
 __global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, const BlockRef* map) {
-  const BlockRef r = map[blockIdx.x];
+  const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_rank;
   advance(d.lvl_filter, r.y, d.base.L);
@@ -1244,7 +1283,7 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, cons
 }
 
 __global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map) {
-  InstDesc& d = desc[map[blockIdx.x].x];
+  InstDesc& d = desc[block_ref(map).x];
   if (d.lvl_filter < d.base.L || d.bt_done) return;  // lvl_filter reaches L after the last rank
   LevelArgs a = resolve(d, d.base.L - 1);
   a.bt_parent = -1;
@@ -1268,21 +1307,32 @@ hipError_t launch_step_init(InstDesc* desc, const BlockRef* map, int nblk, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s) {
-  if (single)
-    hipLaunchKernelGGL(k_expand_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, desc, map);
+// With a timing pair, the launch itself records the events around the
+// kernel's execution (hipExtLaunchKernelGGL), so their interval is the
+// kernel's duration rather than that of separate event packets.
+template <class... A>
+static void launch(void (*k)(A...), dim3 g, dim3 b, size_t lds, hipStream_t s, const EventPair* ev, A... args) {
+  if (ev)
+    hipExtLaunchKernelGGL(k, g, b, (uint32_t)lds, s, ev->start, ev->stop, 0, args...);
   else
-    hipLaunchKernelGGL(k_score_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, desc, map);
+    hipLaunchKernelGGL(k, g, b, lds, s, args...);
+}
+
+hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s,
+                             const EventPair* ev) {
+  launch(single ? k_expand_b : k_score_b, dim3(nblk), dim3(64 * kScoreWaves), 0, s, ev, desc, map);
   return hipGetLastError();
 }
 
-hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s) {
-  hipLaunchKernelGGL(k_filter_b, dim3(nblk), dim3(kFilterThreads), lds_bytes, s, desc, map);
+hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
+                              const EventPair* ev) {
+  launch(k_filter_b, dim3(nblk), dim3(kFilterThreads), lds_bytes, s, ev, desc, map);
   return hipGetLastError();
 }
 
-hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s) {
-  hipLaunchKernelGGL(k_rank_b, dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, desc, map);
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
+                            const EventPair* ev) {
+  launch(k_rank_b, dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
   return hipGetLastError();
 }
 

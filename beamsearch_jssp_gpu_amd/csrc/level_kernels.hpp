@@ -14,7 +14,7 @@ constexpr int kNB = 2048;
 // The score blocks merge their LDS histograms into kHistCopies copies (block
 // b into copy b % kHistCopies), which the filter sums: same-address atomics
 // serialise, so spreading them cuts the merge time by the copy count.
-constexpr int kHistCopies = 16;
+constexpr int kHistCopies = 4;
 // The shift makes the previous level's survivor spread cover ~2^10 bins.
 constexpr int kSpreadBins = 10;
 // Shift used at level 0 (no previous level; at most n children anyway).
@@ -33,6 +33,8 @@ constexpr uint32_t kBitmapWordsMax = 32768;
 struct BlockStat {
   uint32_t valid;           // kept children
   uint32_t expanded;        // children evaluated
+  int min_cmax;             // smallest cmax among the kept children
+  uint32_t pad;
   unsigned long long best;  // last level: min (cmax << 32 | child)
 };
 
@@ -44,7 +46,8 @@ struct RankStat {
 };
 
 struct SelState {
-  uint32_t lo, s1;          // digit parameters of the level (written by score block 0)
+  uint32_t lo, s1;          // digit parameters of the level (exact digit: lo written by the
+                            // previous level's filter block 0; generic: by score block 0)
   uint32_t V, Kp;           // kept children, survivors = min(V, K)
   uint32_t T1, cum_lt, t1, need;  // threshold bin, keys below it, its size, taken from it
   uint32_t sure_ctr, tie_ctr, done_ctr;  // filter counters, zeroed before the level
@@ -62,9 +65,8 @@ struct LevelArgs {
   unsigned long long* keys;  // K*B (also the refinement scratch)
   unsigned long long* crec;  // K*B: kept child's record, finish | machine << 32 | job << 40
   unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
-  unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut
+  unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut / the ties of each filter tile
   uint16_t* kdig;            // K*B: first selection digit of each key
-  uint32_t* tbits;           // ceil(K*B/32) + 2: bitmap of the keys in a single-valued threshold bin
   uint32_t* tcnt;            // ceil(K*B/kFilterThreads): keys of that bin per filter tile
   uint32_t* hist;            // kHistCopies x kNB, this level
   uint32_t* hist_next;       // kHistCopies x kNB, zeroed for the next level
@@ -120,12 +122,21 @@ struct InstDesc {
 typedef int2 BlockRef;
 
 // Step kernels over an array of descriptors; map[b] names the instance and
-// local block of grid block b.
+// local block of grid block b (map null: instance 0, local block b).
 hipError_t launch_step_init(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
-// single: write every kept child's snapshot (single-phase expand).
-hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s);
-hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s);
-hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s);
+// Optional timing events recorded around one launch.
+struct EventPair {
+  hipEvent_t start, stop;
+};
+
+// single: write every kept child's snapshot (single-phase expand).  ev: time
+// this launch (null: plain launch).
+hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s,
+                             const EventPair* ev = nullptr);
+hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
+                              const EventPair* ev = nullptr);
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
+                            const EventPair* ev = nullptr);
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
 
 // Exact (host-replay) modes, driven level by level from the host.

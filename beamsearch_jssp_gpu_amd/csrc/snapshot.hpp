@@ -4,8 +4,9 @@
 // Compact snapshot of one partial schedule: S = 2n + m + 1 int32 words,
 //   [0, m)          machine_ready[k]      finish time of the last op on machine k
 //   [m, m+n)        job_ready[j]          finish time of the last scheduled op of job j
-//   [m+n, m+2n)     job_word[j]           next_op (bits 0-7) | ready position (bits 8-15),
-//                                         position 0xFF once the job is finished
+//   [m+n, m+2n)     job_word[j]           next_op (bits 0-7) | ready position (bits 8-15)
+//                                         | machine of next_op (bits 16-23); position 0xFF
+//                                         and machine 0 once the job is finished
 //   [m+2n]          cmax
 // The ready list of the reference (JobShopState::c, cpu/common/jssp_types.h:30;
 // the precedence_ready block of gpu/common/gpu_snapshot.cuh:20-31) is kept as
@@ -44,9 +45,11 @@ JSSP_HD Layout make_layout(int n, int m) {
   return l;
 }
 
-JSSP_HD int jw_make(int next_op, int pos) { return next_op | (pos << 8); }
+JSSP_HD int jw_make(int next_op, int pos, int mc) { return next_op | (pos << 8) | (mc << 16); }
 JSSP_HD int jw_next(int w) { return w & 0xFF; }
 JSSP_HD int jw_pos(int w) { return (w >> 8) & 0xFF; }
+JSSP_HD int jw_mc(int w) { return (w >> 16) & 0xFF; }
+JSSP_HD int jw_setpos(int w, int pos) { return (w & ~0xFF00) | (pos << 8); }
 
 // Pivot hash, gpu/v1/solver.cu:51-53 (the lowbias32 mixer).
 JSSP_HD uint32_t pivot_hash(uint32_t h) {

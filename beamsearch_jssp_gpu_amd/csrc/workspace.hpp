@@ -20,7 +20,6 @@ struct Work {
   unsigned long long* cand;
   unsigned long long* ref_buf;
   uint16_t* kdig;
-  uint32_t* tbits;
   uint32_t* tcnt;
   uint32_t* hist[2];
   BlockStat* bstat;
@@ -67,15 +66,16 @@ size_t rank_lds_bytes(const Shape& s);
 // Writes (machine, duration) pairs to the staging buffer and copies them to d_ops.
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops);
 
-// Optional per-kernel HIP-event timing on the engine's stream.
+// Optional per-kernel timing: events recorded by the launches themselves.
 struct Prof {
   jssp_engine* e;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> marks;
   size_t next = 0;
   explicit Prof(jssp_engine* eng) : e(eng) {}
   hipEvent_t get();
-  hipEvent_t begin();
-  void end(int kind, hipEvent_t a);
+  // Event pair for one launch of `kind` (stored in *ep), or null when not
+  // profiling or when `on` is false.
+  const EventPair* pair(int kind, EventPair* ep, bool on = true);
   void collect();
 };
 

@@ -4,17 +4,28 @@
 Headline workload: 50 jobs x 20 machines (seeded, durations uniform [1,99]),
 beam width K=4096, operation-level branching (ICH_DET), two-phase path
 (score every child, select the K survivors, materialise only those).  A step
-is one complete solve (1000 levels, ~10^8 expanded states) with the instance
-already uploaded; value = expanded states / second over the timed steps.
+is one complete solve (1000 levels, ~1.65e8 expanded states) with the
+instance already uploaded; value = expanded states / second over the timed
+steps.
 
-Also reported on the same line:
-  roofline      dominant kernel of that solve: algorithmic bytes per launch /
-                mean launch duration (HIP events on the engine's stream, from a
-                profiled solve of the same workload after the timed steps)
-  expand_single_phase   the same for the single-phase expand kernel, which
-                writes every child snapshot (the HBM-bound kernel)
-  cpu_baseline  the reference's own BS-ICH (cpu/v0_ich, compiled into
-                oracle/_ref) on the same instance, one thread, K=1024
+Also reported on the same JSON line:
+  roofline         dominant kernel of that solve: algorithmic bytes per launch
+                   / mean launch duration (HIP events on the engine's stream,
+                   from a profiled solve of the same workload); `traffic` is
+                   the HBM bytes per launch measured with rocprofv3 counters
+                   (read from profiles/traffic_50x20_k4096.json when present)
+  kernels          per-kernel mean launch time and algorithmic GB/s
+  expand_single_phase  the single-phase expand kernel (writes every child)
+  multiseed        16 machine-focused (PMS_GPU) beams sharing one device bound,
+                   as one batch, vs the same 16 solves one after another
+  batched          256 uneven instances in one batched solve vs the plain
+                   one-instance-at-a-time loop (same results checked)
+  serving          Poisson arrivals into the continuous-batching server vs the
+                   plain loop on the same request list: solves/s, p50/p99
+  cpu_baseline     the reference's own BS-ICH (cpu/v0_ich) compiled into
+                   oracle/_ref from the reference sources (or the oracle's
+                   restatement when that library is absent), one thread, one
+                   solve of the same instance at K=2048 (~15 s)
 """
 from __future__ import annotations
 
@@ -22,6 +33,7 @@ import argparse
 import ctypes
 import json
 import os
+import random
 import sys
 import time
 
@@ -30,10 +42,15 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 KIND_NAMES = ["expand_score", "select_filter", "rank_materialize"]
-KERNEL_SYMBOLS = ["k_score", "k_filter", "k_rank"]
+KERNEL_SYMBOLS = {1: ["k_expand_b", "k_filter_b", "k_rank_b"], 2: ["k_score_b", "k_filter_b", "k_rank_b"]}
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "traffic_50x20_k4096.json")
 
 
-def roofline(eng, inst, K, mode, phases, seed):
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def profile_kinds(eng, inst, K, mode, phases, seed):
     eng.set_profiling(True)
     eng.solve(inst, K, mode=mode, phases=phases, seed=seed)
     kinds = []
@@ -44,21 +61,158 @@ def roofline(eng, inst, K, mode, phases, seed):
     return kinds
 
 
-def roofline_obj(kind):
+def roofline_obj(kind, phases, traffic=None):
     k, ms, n, b = kind
     avg_s = ms / 1e3 / max(1, n)
     per_launch = b / max(1, n)
     achieved = per_launch / avg_s / 1e9 if avg_s > 0 else 0.0
-    return {"bound": "hbm", "kernel": KERNEL_SYMBOLS[k], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+    sym = KERNEL_SYMBOLS[phases][k]
+    tr = None
+    if traffic and sym in traffic.get("kernels", {}):
+        tr = traffic["kernels"][sym].get("hbm_bytes_per_launch")
+    return {"bound": "hbm", "kernel": sym, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr,
             "avg_launch_us": round(avg_s * 1e6, 3), "launches": n,
             "algorithmic_bytes_per_launch": round(per_launch, 1)}
 
 
-def cpu_baseline(inst, eng, bs, K=1024, seed=12345):
-    """The reference BS-ICH on this host, one thread, same instance, K=1024.
-    The number of states it expands is the GPU ICH_EXACT solve's count (the
-    same trajectory; its makespan is checked equal)."""
+def load_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def same(a, b):
+    return a.makespan == b.makespan and ((a.sequence is None and b.sequence is None) or
+                                         (a.sequence is not None and b.sequence is not None and
+                                          (a.sequence == b.sequence).all()))
+
+
+def multiseed(eng, bs, inst, K, seed, beams):
+    """16 PMS_GPU beams with a shared device bound, as one batch, vs the same
+    solves one after another with the bound carried the same way (one pass:
+    every beam prunes with the initial bound)."""
+    eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed, n_seeds=beams)  # warm-up
+    t0 = time.perf_counter()
+    r = eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed, n_seeds=beams)
+    t_multi = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    plain = [eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed + i) for i in range(beams)]
+    t_plain = time.perf_counter() - t0
+    best = min(p.makespan for p in plain)
+    return {"workload": f"PMS_GPU 50x20 K={K}, {beams} beams (seeds {seed}..{seed + beams - 1}), one device bound",
+            "beams_per_s": round(beams / t_multi, 2), "plain_beams_per_s": round(beams / t_plain, 2),
+            "speedup": round(t_plain / t_multi, 2), "makespan": r.makespan, "plain_best_makespan": best,
+            "equal": r.makespan == best, "expanded_states_per_s": round(r.expanded_states / t_multi, 1)}
+
+
+def batch_workload(bs, count=256, gen_seed=2026):
+    rng = random.Random(gen_seed)
+    insts, params = [], []
+    for i in range(count):
+        n = rng.choice([10, 15, 20, 30, 50])
+        m = rng.choice([5, 10, 15, 20])
+        K = rng.choice([256, 1024])
+        insts.append(bs.generate_instance(rng.randrange(1, 2**31), n, m, 1, 99))
+        params.append(dict(beam_width=K, mode=bs.MODE_PMS_GPU, phases=2, best_known=bs.NO_MAKESPAN, seed=12345))
+    return insts, params
+
+
+def batched(eng, bs, count):
+    insts, params = batch_workload(bs, count)
+    eng.solve_batch(insts, params)  # warm-up: autotunes the lane shapes, captures the step graph
+    t0 = time.perf_counter()
+    rb = eng.solve_batch(insts, params)
+    t_batch = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    rp = [eng.solve(i, p["beam_width"], mode=p["mode"], seed=p["seed"]) for i, p in zip(insts, params)]
+    t_plain = time.perf_counter() - t0
+    eq = all(same(a, b) for a, b in zip(rb, rp))
+    return {"workload": f"{count} instances, n in {{10,15,20,30,50}}, m in {{5,10,15,20}}, K in {{256,1024}}, "
+                        "durations U[1,99], generator seed 2026, PMS_GPU two-phase",
+            "solves_per_s": round(count / t_batch, 2), "plain_solves_per_s": round(count / t_plain, 2),
+            "speedup": round(t_plain / t_batch, 2), "identical_results": eq,
+            "expanded_states_per_s": round(sum(r.expanded_states for r in rb) / t_batch, 1)}
+
+
+def serve_workload(bs, count, seed=7):
+    rng = random.Random(seed)
+    reqs = []
+    for i in range(count):
+        n = rng.randint(10, 100)
+        m = rng.randint(5, 20)
+        K = rng.choice([256, 1024, 4096])
+        reqs.append((bs.generate_instance(rng.randrange(1, 2**31), n, m, 1, 99), K))
+    gaps = [rng.expovariate(1.0) for _ in range(count)]  # unit-rate Poisson process, scaled below
+    return reqs, gaps
+
+
+def pct(v, q):
+    v = sorted(v)
+    return v[min(len(v) - 1, max(0, int(round(q * (len(v) - 1)))))]
+
+
+def serving(eng, bs, count, budget_gib, load):
+    reqs, gaps = serve_workload(bs, count)
+    # Plain loop: solve times measured one by one; a FIFO server on one GPU.
+    for inst, K in reqs[:2]:
+        eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345)
+    plain, dur = [], []
+    for inst, K in reqs:
+        t0 = time.perf_counter()
+        plain.append(eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345))
+        dur.append(time.perf_counter() - t0)
+    mean_service = sum(dur) / len(dur)
+    # Arrival rate: `load` x the plain loop's capacity.
+    rate = load / mean_service
+    arr, t = [], 0.0
+    for g in gaps:
+        t += g / rate
+        arr.append(t)
+    fin, lat_plain = 0.0, []
+    for a, d in zip(arr, dur):
+        fin = max(fin, a) + d
+        lat_plain.append(fin - a)
+    plain_span = fin - arr[0]
+    # Served: the same arrival times in real time.
+    srv = bs.Server(eng, int(budget_gib * 2**30), max_slots=64)
+    tickets, done, lat = [None] * count, {}, [0.0] * count
+    nxt = 0
+    t0 = time.perf_counter()
+    while len(done) < count:
+        now = time.perf_counter() - t0
+        while nxt < count and arr[nxt] <= now:
+            tickets[nxt] = srv.submit(reqs[nxt][0], reqs[nxt][1], mode=bs.MODE_PMS_GPU, seed=12345)
+            nxt += 1
+        active = srv.step(16)
+        now = time.perf_counter() - t0
+        for i in range(nxt):
+            if i not in done:
+                r = srv.poll(tickets[i])
+                if r is not None:
+                    done[i] = r
+                    lat[i] = now - arr[i]
+        if active == 0 and len(done) == nxt and nxt < count:
+            time.sleep(max(0.0, arr[nxt] - (time.perf_counter() - t0)))
+    served_span = max(arr[i] + lat[i] for i in range(count)) - arr[0]
+    srv.close()
+    eq = all(done[i].makespan == plain[i].makespan and (done[i].sequence == plain[i].sequence).all()
+             for i in range(count))
+    return {"workload": f"{count} requests, Poisson arrivals (seed 7) at {load}x the plain loop's capacity "
+                        f"({rate:.1f}/s), n in [10,100], m in [5,20], K in {{256,1024,4096}}, PMS_GPU, "
+                        f"HBM budget {budget_gib} GiB, 64 slots",
+            "solves_per_s": round(count / served_span, 2), "plain_solves_per_s": round(count / plain_span, 2),
+            "p50_ms": round(pct(lat, 0.5) * 1e3, 2), "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
+            "plain_p50_ms": round(pct(lat_plain, 0.5) * 1e3, 2), "plain_p99_ms": round(pct(lat_plain, 0.99) * 1e3, 2),
+            "identical_results": eq}
+
+
+def cpu_baseline(inst, eng, bs, K=2048, seed=12345):
+    """The reference BS-ICH on this host, one thread, same instance.  The
+    number of states it expands is the GPU ICH_EXACT solve's count (the same
+    trajectory: its makespan is checked equal)."""
     ref_so = os.path.join(REPO, "oracle", "_ref", "libjssp_ref.so")
     n, m = inst.nbjob, inst.nbmachine
     I32P = ctypes.POINTER(ctypes.c_int32)
@@ -101,37 +255,64 @@ def _cpu_model():
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--jobs", type=int, default=50)
     ap.add_argument("--machines", type=int, default=20)
     ap.add_argument("--beam", type=int, default=4096)
+    ap.add_argument("--phases", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-beam", type=int, default=1024)
+    ap.add_argument("--no-extras", action="store_true", help="headline and roofline only")
+    ap.add_argument("--cpu-beam", type=int, default=2048)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--serve-load", type=float, default=2.0)
+    ap.add_argument("--budget-gib", type=float, default=8.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run three device-to-device copies of MIB MiB (reference point for the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        nb = args.copy_calibration * 2**20
+        a = torch.ones(nb // 4, dtype=torch.int32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nb, "copy_bytes_written": nb, "copies": 3}), flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one MI355X: this build runs on a single GPU (independent solves would shard "
-              "one per GPU, see DESIGN.md); --gpus must be 1", file=sys.stderr)
+        print("bench.py measures one MI355X: this build runs on one GPU (independent solves would be spread "
+              "one per GPU); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     import beamsearch_jssp_gpu_amd as bs
     n, m, K, seed = args.jobs, args.machines, args.beam, 12345
     inst = bs.generate_instance(1, n, m, 1, 99)
     eng = bs.Engine(0)
-    mode, phases = bs.MODE_ICH_DET, 2
+    mode, phases = bs.MODE_ICH_DET, args.phases
+    log(f"headline: {n}x{m} K={K} phases={phases}, {args.warmup} warm-up + {args.steps} timed solves")
     for _ in range(args.warmup):
         eng.solve(inst, K, mode=mode, phases=phases, seed=seed)
     t0 = time.perf_counter()
     res = [eng.solve(inst, K, mode=mode, phases=phases, seed=seed) for _ in range(args.steps)]
-    dt = time.perf_counter() - t0
+    wall = time.perf_counter() - t0
+    # The timed region of each solve is its device time (HIP events on the
+    # engine's stream, from the root's init to the sequence backtrack): the
+    # instance is already in HBM then.  The wall-clock rate, which also
+    # includes uploading the op table and descriptor and reading the result
+    # back over PCIe, is reported beside it.
+    dt = sum(r.device_ms for r in res) / 1e3
     states = sum(r.expanded_states for r in res)
     assert len({(r.makespan, tuple(r.sequence)) for r in res}) == 1, "non-deterministic result"
     assert bs.validate_sequence(inst, res[0].sequence) == res[0].makespan
 
-    kinds = roofline(eng, inst, K, mode, phases, seed)
+    log("profiled solve (HIP events per launch)")
+    traffic = load_traffic()
+    kinds = profile_kinds(eng, inst, K, mode, phases, seed)
     dom = max(kinds, key=lambda k: k[1])
-    single = roofline(eng, inst, K, mode, 1, seed)
     line = {
         "metric": "expanded states/s at 50x20 K=4096; batched solves/s + p99 latency vs plain",
         "value": round(states / dt, 1),
@@ -145,17 +326,30 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": f"two-phase operation-level beam search (ICH_DET), {n}x{m}, K={K}, "
-                               f"durations U[1,99], instance seed 1, solve seed {seed}",
+        "value_pcie_inclusive": round(states / wall, 1),
+        "config": {"workload": f"{'two' if phases == 2 else 'single'}-phase operation-level beam search "
+                               f"(ICH_DET), {n}x{m}, K={K}, durations U[1,99], instance seed 1, solve seed {seed}",
                    "jobs": n, "machines": m, "beam_width": K, "levels": n * m,
                    "expanded_states_per_solve": res[0].expanded_states, "makespan": res[0].makespan,
                    "device_ms_per_solve": round(sum(r.device_ms for r in res) / len(res), 3)},
-        "roofline": roofline_obj(dom),
-        "kernels": {KIND_NAMES[k[0]]: {"total_ms": round(k[1], 3), "launches": k[2],
-                                       "avg_us": round(k[1] * 1e3 / max(1, k[2]), 3)} for k in kinds},
-        "expand_single_phase": roofline_obj(single[0]),
+        "roofline": roofline_obj(dom, phases, traffic),
+        "kernels": {KIND_NAMES[k[0]]: {"symbol": KERNEL_SYMBOLS[phases][k[0]], "total_ms": round(k[1], 3),
+                                       "launches": k[2], "avg_us": round(k[1] * 1e3 / max(1, k[2]), 3),
+                                       "algorithmic_GBps": round(k[3] / (k[1] / 1e3) / 1e9, 1) if k[1] else 0}
+                    for k in kinds},
     }
+    if not args.no_extras:
+        if phases == 2:
+            log("single-phase expand kernel")
+            line["expand_single_phase"] = roofline_obj(profile_kinds(eng, inst, K, mode, 1, seed)[0], 1, traffic)
+        log("multi-seed: 16 PMS beams")
+        line["multiseed"] = multiseed(eng, bs, inst, K, 1, 16)
+        log(f"batched: {args.batch} uneven instances")
+        line["batched"] = batched(eng, bs, args.batch)
+        log(f"serving: {args.requests} Poisson requests")
+        line["serving"] = serving(eng, bs, args.requests, args.budget_gib, args.serve_load)
     if not args.no_cpu:
+        log("CPU baseline (reference BS-ICH, one thread)")
         line["cpu_baseline"] = cpu_baseline(inst, eng, bs, K=args.cpu_beam, seed=seed)
     eng.close()
     print(json.dumps(line), flush=True)

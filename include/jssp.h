@@ -123,8 +123,9 @@ int jssp_solve(jssp_engine* engine, const jssp_instance* inst, const jssp_params
 int jssp_solve_trace(jssp_engine* engine, const jssp_instance* inst, const jssp_params* params,
                      jssp_result* result, int32_t* trail, int32_t* pcount);
 
-/* Batched solve: advances all `count` instances together, one fused level
- * kernel per step replayed from a captured hipGraph.  Results equal those of
+/* Batched solve: advances all `count` instances together, one level per
+ * step; a step is three launches (score, filter, rank) whose grids cover every
+ * instance, replayed from a captured hipGraph.  Results equal those of
  * jssp_solve on each instance alone.  Modes ICH_DET and PMS_GPU. */
 int jssp_solve_batch(jssp_engine* engine, int32_t count, const jssp_instance* insts,
                      const jssp_params* params, jssp_result* results);
@@ -146,7 +147,8 @@ int jssp_engine_tuned_lanes(const jssp_engine* engine, int32_t n_jobs, int32_t n
  * switched on (jssp_engine_set_profiling(engine, 1) also resets it), for
  * roofline reporting.  kind: 0 = expand/score (+ first selection
  * histogram), 1 = selection filter, 2 = rank + materialise/gather.
- * total_ms is the sum of HIP-event durations around each launch on the
+ * total_ms is the sum of the kernels' durations, from HIP events that each
+ * launch records around its own execution (hipExtLaunchKernelGGL) on the
  * engine's stream; algorithmic_bytes the bytes the algorithm must move. */
 int jssp_engine_set_profiling(jssp_engine* engine, int32_t on);
 int jssp_engine_kernel_time(const jssp_engine* engine, int32_t kind, double* total_ms,

@@ -21,6 +21,14 @@
 //                   invariant checked on the returned trail, and agreement of
 //                   its ICH-order child generation with orc_ich_exact's.
 //   orc_simulate    cpu/common/jssp_types.h:75-97 applied to a sequence.
+//   orc_pivot_hash  gpu/v1/solver.cu:51-53; pinned by the published inverse
+//                   of lowbias32.
+//   orc_noise_hash  gpu/v1/solver.cu:122-125; pinned by its inverse built from
+//                   the multiplicative inverses of its two odd constants (the
+//                   three xor-shifts by 16 are involutions), which must give
+//                   back seed ^ cid*c1 ^ bid*c2.
+//   orc_gpu_score   gpu/v1/solver.cu:121-127; pinned by exact FP32 values
+//                   computed from the definition in numpy.
 //   orc_bruteforce  exhaustive search over all operation sequences.
 //
 // Float rules: compile without -ffast-math and with -ffp-contract=off; the
@@ -404,5 +412,10 @@ int orc_bruteforce(int n, int m, const int* mach, const int* dur) {
   std::vector<int> mr(m, 0), jr(n, 0), nx(n, 0);
   return bf_rec(n, m, mach, dur, mr, jr, nx, n * m, 0, 0x7fffffff);
 }
+
+// The two hashes, exported for their known-answer / inverse tests.
+uint32_t orc_pivot_hash(uint32_t h) { return pivot_hash(h); }
+uint32_t orc_noise_hash(uint32_t seed, uint32_t cid, uint32_t bid) { return noise_hash(seed, cid, bid); }
+float orc_gpu_score(int cmax, int ns, uint32_t h) { return gpu_score(cmax, ns, h); }
 
 }  // extern "C"

@@ -161,3 +161,51 @@ def test_pruning_semantics():
     assert tight["makespan"] <= free["makespan"]
     none = orc_gpu(inst, 64, PMS_GPU, 1, 1)
     assert not none["found"] and none["makespan"] == 1
+
+
+def test_oracle_pivot_hash_inverts():
+    """The oracle's pivot hash is undone by the published lowbias32 inverse."""
+    lib = oracle()
+    rng = np.random.default_rng(5)
+    for x in [0, 1, 2, 0xFFFFFFFF] + [int(v) for v in rng.integers(0, 2**32, 500)]:
+        assert _lowbias32_inverse(lib.orc_pivot_hash(x)) == x
+
+
+def _unmix_noise(h):
+    # Inverse of: h ^= h>>16; h *= c1; h ^= h>>16; h *= c2; h ^= h>>16.  A
+    # xor-shift by 16 on 32 bits is its own inverse; an odd multiplier is
+    # undone by its inverse mod 2^32.
+    c1i, c2i = pow(0x85EBCA6B, -1, 2**32), pow(0xC2B2AE35, -1, 2**32)
+    h ^= h >> 16
+    h = (h * c2i) & 0xFFFFFFFF
+    h ^= h >> 16
+    h = (h * c1i) & 0xFFFFFFFF
+    h ^= h >> 16
+    return h
+
+
+def test_oracle_noise_hash_inverts_to_its_input_mix():
+    """The sibling-noise hash (gpu/v1/solver.cu:122-125): unmixing its output
+    gives back seed ^ cid*c1 ^ bid*c2; a wrong shift, multiplier or a swapped
+    cid/bid breaks it."""
+    lib = oracle()
+    rng = np.random.default_rng(6)
+    for _ in range(500):
+        seed, cid, bid = (int(v) for v in rng.integers(0, 2**32, 3))
+        bid %= 128
+        h = lib.orc_noise_hash(seed, cid, bid)
+        assert _unmix_noise(h) == (seed ^ ((cid * 0x85EBCA6B) & 0xFFFFFFFF) ^ ((bid * 0xC2B2AE35) & 0xFFFFFFFF))
+    # cid and bid are not interchangeable
+    assert lib.orc_noise_hash(7, 3, 5) != lib.orc_noise_hash(7, 5, 3)
+
+
+def test_oracle_score_is_fp32_division_plus_noise():
+    """score = fl(fl(cmax / ns) + fl((h % 1000) / 1e9)) in FP32, in that
+    order; compared with numpy's IEEE float32 arithmetic."""
+    lib = oracle()
+    rng = np.random.default_rng(7)
+    for _ in range(2000):
+        cmax, ns, h = int(rng.integers(0, 200000)), int(rng.integers(1, 33000)), int(rng.integers(0, 2**32))
+        want = np.float32(np.float32(cmax) / np.float32(ns)) + np.float32(np.float32(h % 1000) / np.float32(1e9))
+        got = np.float32(lib.orc_gpu_score(cmax, ns, h))
+        assert got.view(np.uint32) == np.float32(want).view(np.uint32), (cmax, ns, h)

@@ -41,6 +41,12 @@ def oracle():
                                       ctypes.c_int, ctypes.c_uint, I32P, ll, ip, I32P, I32P]
         lib.orc_simulate.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, I32P]
         lib.orc_bruteforce.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P]
+        lib.orc_pivot_hash.argtypes = [ctypes.c_uint32]
+        lib.orc_pivot_hash.restype = ctypes.c_uint32
+        lib.orc_noise_hash.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        lib.orc_noise_hash.restype = ctypes.c_uint32
+        lib.orc_gpu_score.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32]
+        lib.orc_gpu_score.restype = ctypes.c_float
         _oracle = lib
     return _oracle
 
