@@ -33,6 +33,8 @@ MODE_ICH_EXACT, MODE_ICH_DET, MODE_PMS_GPU, MODE_PMS_EXACT = 0, 1, 2, 3
 FLAG_GLOBAL_RAND = 0x1
 FLAG_RADIX_CUT = 0x2
 FLAG_GENERIC_DIGIT = 0x4
+FLAG_UNFUSED = 0x8
+LANES_FUSED = 1
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 

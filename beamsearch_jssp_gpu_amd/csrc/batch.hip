@@ -47,6 +47,7 @@ struct BatchPlan {
   InstDesc* d_desc = nullptr;
   BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_inst = nullptr;
   int nms = 0, nmf = 0, nmr = 0;
+  int n_variant[kRankVariants] = {};  // instances per rank kernel variant
   size_t flds = 0, rlds = 0;
   int* d_res = nullptr;                 // count x 4
   unsigned long long* d_exp = nullptr;  // count
@@ -97,6 +98,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     seq_total += (size_t)s.L;
     bp.Lmax = std::max(bp.Lmax, s.L);
     append_maps(s, i, ms, mf, mr);
+    bp.n_variant[rank_variant(s, &ps[i])]++;
     mi.push_back(make_int2(i, 0));
     bp.flds = std::max(bp.flds, filter_lds_bytes(s, &ps[i]));
     bp.rlds = std::max(bp.rlds, rank_lds_bytes(s));
@@ -182,9 +184,11 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
 }
 
 int enqueue_step(jssp_engine* e, const BatchPlan& bp) {
-  JSSP_HIP(e, launch_step_score(bp.d_desc, bp.d_ms, bp.nms, false, e->stream));
+  if (bp.nms > 0) JSSP_HIP(e, launch_step_score(bp.d_desc, bp.d_ms, bp.nms, false, e->stream));
   JSSP_HIP(e, launch_step_filter(bp.d_desc, bp.d_mf, bp.nmf, bp.flds, e->stream));
-  JSSP_HIP(e, launch_step_rank(bp.d_desc, bp.d_mr, bp.nmr, bp.rlds, e->stream));
+  // Each rank kernel variant skips the other variants' instances.
+  for (int v = 0; v < kRankVariants; ++v)
+    if (bp.n_variant[v] > 0) JSSP_HIP(e, launch_step_rank(bp.d_desc, bp.d_mr, bp.nmr, bp.rlds, v, e->stream));
   return JSSP_OK;
 }
 
@@ -200,7 +204,9 @@ int enqueue_steps(jssp_engine* e, const BatchPlan& bp, int steps, bool use_graph
   const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)bp.d_desc, (uint64_t)bp.nms, (uint64_t)bp.nmf,
                                      (uint64_t)bp.nmr, (uint64_t)bp.flds, (uint64_t)bp.rlds,
                                      (uint64_t)(uintptr_t)bp.d_ms, (uint64_t)(uintptr_t)bp.d_mf,
-                                     (uint64_t)(uintptr_t)bp.d_mr};
+                                     (uint64_t)(uintptr_t)bp.d_mr, (uint64_t)bp.n_variant[0],
+                                     (uint64_t)bp.n_variant[1], (uint64_t)bp.n_variant[2],
+                                     (uint64_t)bp.n_variant[3], (uint64_t)bp.n_variant[4]};
   if (!e->step_graph || e->step_graph_key != key) {
     if (e->step_graph) {
       JSSP_HIP(e, hipGraphExecDestroy(e->step_graph));
@@ -273,7 +279,7 @@ void load_tune_file(jssp_engine* e) {
   int n, m, kb, g;
   while (f >> n >> m >> kb >> g) {
     const uint64_t key = ((uint64_t)n << 40) | ((uint64_t)m << 32) | (uint64_t)kb;
-    if (!lookup_tune(e, key) && (g == 16 || g == 32 || g == 64)) e->tune.push_back({key, g});
+    if (!lookup_tune(e, key) && (g == kLanesFused || g == 16 || g == 32 || g == 64)) e->tune.push_back({key, g});
   }
 }
 
@@ -307,8 +313,10 @@ int autotune_classes(jssp_engine* e, int count, const jssp_instance* insts, cons
     const int steps = std::min(24, n * m);
     int best_g = 64;
     double best_t = 1e30;
-    for (int g : {16, 32, 64}) {
-      if (g < 64 && g < n) continue;
+    const bool can_fuse = fused_eligible(&insts[i], &params[i]);
+    for (int g : {kLanesFused, 16, 32, 64}) {
+      if (g == kLanesFused && !can_fuse) continue;
+      if (g != kLanesFused && g < 64 && g < n) continue;
       std::vector<int> lanes(sub.size(), g);
       BatchPlan bp;
       int rc = plan_batch(e, (int)sub.size(), sub.data(), subp.data(), lanes.data(), false, bp);

@@ -102,7 +102,24 @@ int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p) 
 // Workspace planning.
 int default_lanes(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int G) {
+// Dynamic LDS of the rank kernel: survivor keys (K <= kRankLdsMax), the tile
+// prefix of tie counts (padded to 16 bytes) and, fused, one snapshot view per
+// wave.
+static size_t rank_lds_for(int K, int n, int m, bool fused) {
+  const size_t tiles = ((size_t)K * n + kFilterThreads - 1) / kFilterThreads;
+  size_t b = (K <= kRankLdsMax ? (size_t)K * 8 : 0) + 4 * ((tiles + 1 + 3) & ~(size_t)3);
+  if (fused) b += (size_t)kRankWaves * (2 * n + m + 1) * 4;
+  return b;
+}
+
+bool fused_eligible(const jssp_instance* in, const jssp_params* p) {
+  const bool exact = (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT);
+  return !exact && p->phases == 2 && !(p->flags & (JSSP_FLAG_GENERIC_DIGIT | JSSP_FLAG_UNFUSED)) &&
+         rank_lds_for(p->beam_width, in->n_jobs, in->n_machines, true) <= kRankFusedLdsMax &&
+         exact_digit_applies(in);
+}
+
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes) {
   Shape s;
   s.n = in->n_jobs; s.m = in->n_machines;
   s.S = 2 * s.n + s.m + 1;
@@ -111,8 +128,10 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int G) {
   s.L = s.n * s.m;
   s.exact = (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT);
   s.single = !s.exact && p->phases == 1;
+  s.fused = (lanes == 0 || lanes == kLanesFused) && fused_eligible(in, p);
+  int G = lanes;
   if (G != 16 && G != 32 && G != 64) G = default_lanes(s.n);
-  if (G < s.n || s.single) G = 64;
+  if (G < s.n || s.single || s.fused) G = 64;
   s.G = G;
   const long long N = (long long)s.K * s.B;
   const int parents_per_block = kScoreWaves * (64 / s.G);
@@ -131,14 +150,15 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.beam[1] = b.take<int>((size_t)s.K * s.S);
   w.cbuf = s.single ? b.take<int>(N * s.S) : nullptr;
   w.keys = b.take<unsigned long long>(std::max(N, (size_t)2 * s.K));
-  w.crec = b.take<unsigned long long>(N);
+  w.crec[0] = b.take<unsigned long long>(N);
+  w.crec[1] = b.take<unsigned long long>(N);
   w.cand = b.take<unsigned long long>(N);
   w.ref_buf = b.take<unsigned long long>(N + s.K);
   w.kdig = b.take<uint16_t>(N);
   w.tcnt = b.take<uint32_t>((N + kFilterThreads - 1) / kFilterThreads + 2);
   w.hist[0] = b.take<uint32_t>((size_t)kNB * kHistCopies);
   w.hist[1] = b.take<uint32_t>((size_t)kNB * kHistCopies);
-  w.bstat = b.take<BlockStat>(s.nblk_score);
+  w.bstat = b.take<BlockStat>(std::max(s.nblk_score, s.nblk_rank));
   w.rstat = b.take<RankStat>(s.nblk_rank);
   w.sel[0] = b.take<SelState>(1);
   w.sel[1] = b.take<SelState>(1);
@@ -186,7 +206,7 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.beam_next = w.beam[1];
   a.cbuf = w.cbuf;
   a.keys = w.keys;
-  a.crec = w.crec;
+  a.crec = w.crec[0];
   a.cand = w.cand;
   a.ref_buf = w.ref_buf;
   a.kdig = w.kdig;
@@ -229,9 +249,14 @@ InstDesc make_desc(const Shape& s, const Work& w, const jssp_params* p, const js
   d.beam[0] = w.beam[0]; d.beam[1] = w.beam[1];
   d.hist[0] = w.hist[0]; d.hist[1] = w.hist[1];
   d.sel[0] = w.sel[0]; d.sel[1] = w.sel[1];
+  d.crec[0] = w.crec[0]; d.crec[1] = w.crec[1];
   d.nblk_score = s.nblk_score;
   d.nblk_filter = s.nblk_filter;
   d.nblk_rank = s.nblk_rank;
+  d.fused = s.fused ? 1 : 0;
+  // Score partials per level: one per score block, or one per rank block
+  // when the rank kernel scores the next level.
+  d.base.nblk_score = s.fused ? s.nblk_rank : s.nblk_score;
   d.lvl_score = 0;
   d.lvl_filter = 0;
   d.lvl_rank = -1;
@@ -240,7 +265,8 @@ InstDesc make_desc(const Shape& s, const Work& w, const jssp_params* p, const js
 
 void append_maps(const Shape& s, int inst, std::vector<BlockRef>& score, std::vector<BlockRef>& filter,
                  std::vector<BlockRef>& rank) {
-  for (int b = 0; b < s.nblk_score; ++b) score.push_back(make_int2(inst, b));
+  if (!s.fused)
+    for (int b = 0; b < s.nblk_score; ++b) score.push_back(make_int2(inst, b));
   for (int b = 0; b < s.nblk_filter; ++b) filter.push_back(make_int2(inst, b));
   for (int b = 0; b < s.nblk_rank; ++b) rank.push_back(make_int2(inst, b));
 }
@@ -250,10 +276,13 @@ size_t filter_lds_bytes(const Shape& s, const jssp_params* p) {
   return 4 * std::min<size_t>(((size_t)s.K * s.B + 31) / 32, kBitmapWordsMax);
 }
 
-// Survivor keys (K <= kRankLdsMax) and the prefix of per-tile tie counts.
-size_t rank_lds_bytes(const Shape& s) {
-  const size_t tiles = ((size_t)s.K * s.B + kFilterThreads - 1) / kFilterThreads;
-  return (s.K <= kRankLdsMax ? (size_t)s.K * 8 : 0) + 4 * (tiles + 1);
+size_t rank_lds_bytes(const Shape& s) { return rank_lds_for(s.K, s.n, s.m, s.fused); }
+
+int rank_variant(const Shape& s, const jssp_params* p) {
+  if (!s.fused) return kRankUnfused;
+  const bool small = s.S <= 128 && s.n <= 64;
+  if (p->mode == JSSP_MODE_PMS_GPU) return small ? kRankFusedPmsSmall : kRankFusedPms;
+  return small ? kRankFusedIchSmall : kRankFusedIch;
 }
 
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops) {
@@ -304,14 +333,16 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
   // materialisation reads a parent and the survivor's 8-byte record and
   // writes a child snapshot.
   const double S4 = 4.0 * s.S;
+  // Fused: the rank kernel of level l also scores level l + 1 from the
+  // survivors' words in registers (8 bytes per child key, no parent read).
   for (int l = 0; l < s.L; ++l) {
     const double P = pc[l], C = lexp[l];
     const double Kp = (double)pc[l + 1];
     if (s.single) e->kt[kKindScore].bytes += S4 * P + (S4 + 8.0) * C;
-    else e->kt[kKindScore].bytes += S4 * P + 8.0 * C;
+    else if (!s.fused) e->kt[kKindScore].bytes += S4 * P + 8.0 * C;
     if (l == s.L - 1) break;
     e->kt[kKindFilter].bytes += 8.0 * C + 8.0 * Kp;
-    e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 8.0 * Kp;
+    e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 8.0 * Kp + (s.fused ? 8.0 * lexp[l + 1] : 0.0);
   }
 }
 
@@ -347,11 +378,12 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   JSSP_HIP(e, launch_step_init(d_desc, nullptr, 1, e->stream));
   for (int l = 0; l < s.L; ++l) {
     EventPair ep;
-    JSSP_HIP(e, launch_step_score(d_desc, nullptr, s.nblk_score, s.single, e->stream,
-                                  prof.pair(kKindScore, &ep)));
+    if (!s.fused)
+      JSSP_HIP(e, launch_step_score(d_desc, nullptr, s.nblk_score, s.single, e->stream,
+                                    prof.pair(kKindScore, &ep)));
     JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, e->stream,
                                    prof.pair(kKindFilter, &ep, l < s.L - 1)));
-    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, e->stream,
+    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, rank_variant(s, &q), e->stream,
                                  prof.pair(kKindRank, &ep, l < s.L - 1)));
   }
   JSSP_HIP(e, launch_step_backtrack(d_desc, nullptr, 1, e->stream));

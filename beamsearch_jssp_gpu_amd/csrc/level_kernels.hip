@@ -48,6 +48,16 @@ namespace {
 
 typedef unsigned long long u64;
 
+// Global-memory (address space 1) view of a pointer.  The descriptors hold
+// generic pointers; accessing through a typed view lets the compiler emit
+// global_* instead of flat_* instructions (flat ones also wait on the LDS
+// counter).
+#define GL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ GL T* gp(T* p) {
+  return (GL T*)p;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -168,7 +178,8 @@ __device__ __forceinline__ int rec_job(u64 r) { return (int)((r >> 40) & 0xFFu);
 __device__ __forceinline__ int rec_next_machine(u64 r) { return (int)((r >> 48) & 0xFFu); }
 
 // Machine of op k+1 of job j (0 past the last op).
-__device__ __forceinline__ int next_machine(const int2* ops, int j, int m, int k) {
+template <class P>
+__device__ __forceinline__ int next_machine(P ops, int j, int m, int k) {
   return k + 1 < m ? ops[j * m + k + 1].x : 0;
 }
 
@@ -232,41 +243,307 @@ __global__ void k_init(LevelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Score (two-phase) or expand (single-phase).  A group of G lanes (G = 16,
-// 32 or 64, G >= n unless G = 64) handles one parent; lane j of the group
-// holds job j (and j + 64 when n > 64).
-// Mode 1 = ICH_DET (every ready op, slot = ready position),
-// mode 2 = PMS_GPU (ops on the hash-chosen pivot machine, gpu/v1/solver.cu:47-70).
-template <bool SINGLE>
-__device__ void score_body(const LevelArgs& a, int local, int nblk) {
-  __shared__ uint32_t s_h[kNB];
+// Scoring of one parent's children, shared by the score kernels, the root
+// (init) and the rank kernel's fused scoring of the next level.
+
+// Per-wave partials of a score pass.
+struct ScoreAcc {
+  uint32_t cnt = 0;   // kept children counted in the histogram
+  uint32_t exp = 0;   // children evaluated
+  int kmin = INT_MAX; // smallest kept cmax
+  u64 best = ~0ull;   // last level: min (cmax << 32 | child)
+};
+
+// One parent as seen by its lane group: lane j of the group holds job j in
+// slot h = 0 and job j + 64 in slot h = 1 (n > 64, G = 64 only).
+struct JobView {
+  bool rdy[2];
+  int pos[2], mc[2], f[2], cm[2], nx[2], nmc[2];
+  int r;  // ready jobs of the parent
+};
+
+// Where a score pass writes, and the per-level constants it needs.
+struct ScoreDst {
+  GL u64* keys;
+  GL uint16_t* kdig;
+  GL u64* crec;
+  uint32_t lseed;
+  int B, mode, exact_digit, best_known;
+};
+__device__ __forceinline__ ScoreDst score_dst(const LevelArgs& a) {
+  ScoreDst d;
+  d.keys = gp(a.keys);
+  d.kdig = gp(a.kdig);
+  d.crec = gp(a.crec);
+  d.lseed = a.lseed;
+  d.B = a.B;
+  d.mode = a.mode;
+  d.exact_digit = a.exact_digit;
+  d.best_known = a.best_known;
+  return d;
+}
+
+// Digit and level parameters of a score pass.
+struct ScoreLevel {
+  uint32_t lo, s1;
+  int ns;     // operations scheduled in a child (level + 1)
+  bool last;  // last level: only the best complete child is recorded
+};
+
+// Word o (per lane) of a snapshot held by the wave as w[i] = word lane + 64 i.
+// Every lane of the wave must execute it.
+__device__ __forceinline__ int wave_fetch(const int (&w)[kMaxWPL], int NW, int o) {
+  int v = 0;
+#pragma unroll
+  for (int i = 0; i < kMaxWPL; ++i) {
+    if (i < NW) {
+      const int t = __shfl(w[i], o & 63);
+      v = ((o >> 6) == i) ? t : v;
+    }
+  }
+  return v;
+}
+
+// The job view of a parent whose words the wave holds (G = 64); cmax = the
+// parent's cmax.  Every lane of the wave must execute it.
+__device__ __forceinline__ JobView view_from_words(GL const int2* ops, const Layout& Ly, const int (&w)[kMaxWPL],
+                                                   int NW, int cmax, int lane) {
+  JobView v;
+  const int n = Ly.n, m = Ly.m;
+  const int NH = (n + 63) >> 6;
+  v.r = 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    v.rdy[h] = false; v.pos[h] = kFinishedPos; v.mc[h] = 0; v.f[h] = 0; v.cm[h] = 0; v.nx[h] = 0; v.nmc[h] = 0;
+    if (h < NH) {  // wave-uniform
+      const int j = h * 64 + lane;
+      const bool in = j < n;
+      const int jw = wave_fetch(w, NW, in ? Ly.off_jw + j : 0);
+      // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
+      const int pidx = (m == 1) ? (j > 0 ? Ly.off_jr + j - 1 : -1) : Ly.off_jr + j;
+      int pred = wave_fetch(w, NW, (in && pidx >= 0) ? pidx : 0);
+      pred = (in && pidx >= 0) ? pred : 0;
+      const int mc = jw_mc(jw);
+      const int mr = wave_fetch(w, NW, in ? Ly.off_mr + mc : 0);
+      if (in) {
+        v.pos[h] = jw_pos(jw);
+        if (v.pos[h] != kFinishedPos) {
+          v.rdy[h] = true;
+          v.nx[h] = jw_next(jw);
+          v.mc[h] = mc;
+          const int dur = ops[j * m + v.nx[h]].y;
+          v.nmc[h] = next_machine(ops, j, m, v.nx[h]);
+          v.f[h] = max(mr, pred) + dur;
+          v.cm[h] = max(cmax, v.f[h]);
+        }
+      }
+      v.r += __popcll(__ballot(v.rdy[h]));
+    }
+  }
+  return v;
+}
+
+// The job view of a parent whose words the wave has written to LDS at vw
+// (G = 64); cmax = the parent's cmax.
+// NHM: job slots per lane the caller supports (1 when n <= 64).
+template <int NHM>
+__device__ __forceinline__ JobView view_from_lds(GL const int2* ops, const Layout& Ly, const int* vw, int cmax,
+                                                 int lane) {
+  JobView v;
+  const int n = Ly.n, m = Ly.m;
+  const int NH = (n + 63) >> 6;
+  v.r = 0;
+  v.rdy[1] = false; v.pos[1] = kFinishedPos; v.mc[1] = 0; v.f[1] = 0; v.cm[1] = 0; v.nx[1] = 0; v.nmc[1] = 0;
+#pragma unroll
+  for (int h = 0; h < NHM; ++h) {
+    v.rdy[h] = false; v.pos[h] = kFinishedPos; v.mc[h] = 0; v.f[h] = 0; v.cm[h] = 0; v.nx[h] = 0; v.nmc[h] = 0;
+    if (h < NH) {  // wave-uniform
+      const int j = h * 64 + lane;
+      if (j < n) {
+        const int jw = vw[Ly.off_jw + j];
+        // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
+        const int pred = (m == 1) ? (j > 0 ? vw[Ly.off_jr + j - 1] : 0) : vw[Ly.off_jr + j];
+        v.pos[h] = jw_pos(jw);
+        if (v.pos[h] != kFinishedPos) {
+          v.rdy[h] = true;
+          v.nx[h] = jw_next(jw);
+          v.mc[h] = jw_mc(jw);
+          const int dur = ops[j * m + v.nx[h]].y;
+          v.nmc[h] = next_machine(ops, j, m, v.nx[h]);
+          v.f[h] = max(vw[Ly.off_mr + v.mc[h]], pred) + dur;
+          v.cm[h] = max(cmax, v.f[h]);
+        }
+      }
+      v.r += __popcll(__ballot(v.rdy[h]));
+    }
+  }
+  return v;
+}
+
+// Children of parent p (lane group of G lanes, gl = lane in the group, gmask
+// = the group's lanes): branch slots (ICH_DET: ready position; PMS_GPU: rank
+// among the hash-chosen pivot machine's ops, gpu/v1/solver.cu:47-70), pruning
+// against best_known, the FP32 score and 64-bit key, the first selection
+// digit, the child record, the LDS histogram; invalid slots get key ~0.  The
+// whole wave executes it; lanes of an inactive parent (pact false) hold no
+// ready job.  Fills slot / keep / nvalid for the single-phase writer.
+// NHM: job slots per lane the caller supports (1 when n <= 64: the view's
+// second slot is then empty).
+template <int NHM = 2>
+__device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLevel& sl, int p, bool pact, int G,
+                                              int gl, u64 gmask, int lane, JobView& v, uint32_t* s_h,
+                                              ScoreAcc& acc, int (&slot)[2], bool (&keep)[2], int& nvalid) {
+  const int B = a.B;
+  bool val[2];
+  slot[1] = 0;
+  val[1] = false;
+  keep[1] = false;
+  if (a.mode == 1) {
+#pragma unroll
+    for (int h = 0; h < NHM; ++h) {
+      slot[h] = v.pos[h];
+      val[h] = v.rdy[h];
+    }
+    nvalid = v.r;
+  } else {
+    // Pivot: ready[hash % r] (gpu/v1/solver.cu:51-56); every lane reads its
+    // own group's source lane.
+    const uint32_t hsh = pivot_hash(a.lseed + (uint32_t)p);
+    const int tpos = v.r > 0 ? (int)(hsh % (uint32_t)v.r) : -1;
+    const u64 b0 = __ballot(v.rdy[0] && v.pos[0] == tpos) & gmask;
+    const int t0 = __shfl(v.mc[0], b0 ? __ffsll((long long)b0) - 1 : lane);
+    int tmc = b0 ? t0 : -1;
+    if (NHM > 1) {
+      const u64 b1 = __ballot(v.rdy[1] && v.pos[1] == tpos) & gmask;
+      const int t1 = __shfl(v.mc[1], b1 ? __ffsll((long long)b1) - 1 : lane);
+      tmc = b0 ? t0 : (b1 ? t1 : -1);
+    }
+#pragma unroll
+    for (int h = 0; h < NHM; ++h) val[h] = v.rdy[h] && v.mc[h] == tmc;
+    // Slot = rank of the ready position among the pivot machine's ops:
+    // the group's candidate positions as a 128-bit mask.
+    u64 plo = 0, phi = 0;
+#pragma unroll
+    for (int h = 0; h < NHM; ++h) {
+      if (val[h]) {
+        if (v.pos[h] < 64) plo |= 1ull << v.pos[h];
+        else phi |= 1ull << (v.pos[h] - 64);
+      }
+    }
+    for (int o = 1; o < G; o <<= 1) {
+      plo |= __shfl_xor(plo, o);
+      if (NHM > 1) phi |= __shfl_xor(phi, o);
+    }
+#pragma unroll
+    for (int h = 0; h < NHM; ++h) {
+      const int ps = v.pos[h];
+      slot[h] = ps < 64 ? __popcll(plo & ((1ull << ps) - 1ull))
+                        : __popcll(plo) + __popcll(phi & ((1ull << (ps - 64)) - 1ull));
+    }
+    nvalid = __popcll(plo) + __popcll(phi);
+  }
+  if (pact && gl == 0) acc.exp += (uint32_t)nvalid;
+
+#pragma unroll
+  for (int h = 0; h < NHM; ++h) {
+    val[h] = val[h] && slot[h] < B;  // always true for a consistent snapshot; keeps writes in bounds
+    keep[h] = val[h] && v.cm[h] < a.best_known;
+    uint32_t d = 0;
+    bool counted = false;
+    if (val[h]) {
+      const uint32_t x = (uint32_t)(p * B + slot[h]);
+      const int j = h * 64 + gl;
+      if (keep[h]) {
+        a.crec[x] = pack_rec(v.f[h], v.mc[h], j, v.nmc[h]);
+        if (sl.last) {
+          const u64 c = ((u64)(uint32_t)v.cm[h] << 32) | x;
+          acc.best = c < acc.best ? c : acc.best;
+        } else {
+          const float base = gpu_score_base(v.cm[h], sl.ns);
+          const float sc = gpu_score_add_noise(base, noise_hash(a.lseed, (uint32_t)p, (uint32_t)slot[h]));
+          const uint32_t bits = __float_as_uint(sc);
+          a.keys[x] = ((u64)bits << 32) | x;
+          if (a.exact_digit) {
+            // Monotone in (score, child) when a lower cmax always scores
+            // lower (checked per instance on the host): cmax >= lo, and for
+            // one cmax the score grows with its ulp offset.
+            const uint32_t dc = min((uint32_t)(v.cm[h] - (int)sl.lo), 256u);
+            const uint32_t off = min(bits - __float_as_uint(base), 7u);
+            d = min(8u * dc + off, (uint32_t)(kNB - 1));
+          } else {
+            d = digit1(bits, sl.lo, sl.s1);
+          }
+          a.kdig[x] = (uint16_t)d;
+          counted = true;
+          acc.cnt++;
+          acc.kmin = min(acc.kmin, v.cm[h]);
+        }
+      } else if (!sl.last) {
+        a.keys[x] = ~0ull;
+      }
+    }
+    lds_count(s_h, d, counted, lane);
+  }
+  if (!sl.last && pact) {
+    for (int s = gl; s < B; s += G)
+      if (s >= nvalid) a.keys[(size_t)p * B + s] = ~0ull;
+  }
+}
+
+// Block partials of a score pass: bstat[slot] and the LDS histogram merged
+// into copy `slot % kHistCopies` of hist.  Every thread of the block calls it.
+__device__ __forceinline__ void finish_score_block(ScoreAcc acc, const uint32_t* s_h, BlockStat* bstat,
+                                                   GL uint32_t* hist, int slot, bool last) {
   __shared__ uint32_t s_cnt[16], s_exp[16];
   __shared__ int s_kmin[16];
   __shared__ u64 s_best[16];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nwaves = (int)(blockDim.x >> 6);
+  acc.cnt = wave_sum(acc.cnt);
+  acc.exp = wave_sum(acc.exp);
+  acc.kmin = wave_min_i(acc.kmin);
+  acc.best = wave_min64(acc.best);
+  if (lane == 0) { s_cnt[wave] = acc.cnt; s_exp[wave] = acc.exp; s_kmin[wave] = acc.kmin; s_best[wave] = acc.best; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    BlockStat b;
+    b.valid = 0; b.expanded = 0; b.min_cmax = INT_MAX; b.pad = 0; b.best = ~0ull;
+    for (int w = 0; w < nwaves; ++w) {
+      b.valid += s_cnt[w];
+      b.expanded += s_exp[w];
+      b.min_cmax = min(b.min_cmax, s_kmin[w]);
+      b.best = s_best[w] < b.best ? s_best[w] : b.best;
+    }
+    bstat[slot] = b;
+  }
+  if (!last) {
+    GL uint32_t* hc = hist + (size_t)(slot % kHistCopies) * kNB;
+    for (int i = threadIdx.x; i < kNB; i += blockDim.x)
+      if (s_h[i]) __hip_atomic_fetch_add(&hc[i], s_h[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Digit parameters of a level.  Exact digit: lo = a lower bound of every
+// parent's cmax, the smallest kept child's cmax of the previous level
+// (written by that level's filter; 0 for the root).  Generic digit: every
+// child's score is >= fl(cmin / ns), cmin = the smallest parent cmax: cmax
+// only grows, correctly rounded division is monotone, and adding the
+// non-negative noise cannot round below the base; the shift spreads the
+// previous level's survivors' score range over ~2^kSpreadBins bins (from the
+// rank kernel's partials).  Every thread of the block calls it.
+__device__ __forceinline__ ScoreLevel score_level(const LevelArgs& a, int local) {
   __shared__ int s_rmin[16];
   __shared__ uint32_t s_rmax[16], s_rlo[16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwaves = (int)(blockDim.x >> 6);
-  const Layout Ly = make_layout(a.n, a.m);
-  const int n = a.n, m = a.m, S = Ly.S, B = a.B;
-  const int P = a.pcount[a.level];
-  const bool last = (a.level == a.L - 1);
-  const int ns = a.level + 1;
-  const int NW = (S + 63) >> 6;
-  const int mode = a.mode;
-  constexpr bool single = SINGLE;
-
-  // Digit parameters.  Exact digit: lo = a lower bound of every parent's
-  // cmax, the smallest kept child's cmax of the previous level (written by
-  // that level's filter; 0 for the root).  Generic digit: every child's score
-  // is >= fl(cmin / ns), cmin = the smallest parent cmax: cmax only grows,
-  // correctly rounded division is monotone, and adding the non-negative noise
-  // cannot round below the base; the shift spreads the previous level's
-  // survivors' score range over ~2^kSpreadBins bins.
-  uint32_t lo = 0, s1 = kLevel0Shift;
+  ScoreLevel sl;
+  sl.ns = a.level + 1;
+  sl.last = (a.level == a.L - 1);
+  sl.lo = 0;
+  sl.s1 = kLevel0Shift;
   if (a.exact_digit) {
-    lo = a.sel->lo;
-  } else if (a.level > 0 && !last) {
+    sl.lo = a.sel->lo;
+  } else if (a.level > 0 && !sl.last) {
     int cmin = INT_MAX;
     uint32_t mxh = 0, mnh = 0xFFFFFFFFu;
     for (int i = threadIdx.x; i < a.nblk_rank; i += blockDim.x) {
@@ -287,16 +564,35 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
       mxh = max(mxh, s_rmax[w]);
       mnh = min(mnh, s_rlo[w]);
     }
-    if (cmin != INT_MAX) lo = __float_as_uint((float)cmin / (float)ns);
+    if (cmin != INT_MAX) sl.lo = __float_as_uint((float)cmin / (float)sl.ns);
     const uint32_t spread = mxh > mnh ? mxh - mnh : 0u;
     const int bl = spread ? 32 - __clz(spread) : 0;
-    s1 = (uint32_t)max(0, bl - kSpreadBins);
+    sl.s1 = (uint32_t)max(0, bl - kSpreadBins);
   }
   if (local == 0 && threadIdx.x == 0) {
-    if (!a.exact_digit) a.sel->lo = lo;
-    a.sel->s1 = s1;
+    if (!a.exact_digit) a.sel->lo = sl.lo;
+    a.sel->s1 = sl.s1;
     a.sel->exact = a.exact_digit ? 1u : 0u;
   }
+  return sl;
+}
+
+// Score (two-phase) or expand (single-phase) kernel body.  A group of G lanes
+// (G = 16, 32 or 64, G >= n unless G = 64) handles one parent read from the
+// beam in HBM; lane j of the group holds job j (and j + 64 when n > 64).
+// Mode 1 = ICH_DET (every ready op, slot = ready position), mode 2 = PMS_GPU.
+template <bool SINGLE>
+__device__ void score_body(const LevelArgs& a, int local, int nblk) {
+  __shared__ uint32_t s_h[kNB];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nwaves = (int)(blockDim.x >> 6);
+  const Layout Ly = make_layout(a.n, a.m);
+  const int n = a.n, m = a.m, S = Ly.S, B = a.B;
+  const int P = a.pcount[a.level];
+  const int NW = (S + 63) >> 6;
+  const ScoreLevel sl = score_level(a, local);
+  const ScoreDst dst = score_dst(a);
+  GL const int2* ops = gp(a.ops);
   for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
   __syncthreads();
 
@@ -306,147 +602,58 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   const int sub = lane >> lg, gl = lane & (G - 1);
   const u64 gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (sub * G));
   const int NH = (G == 64) ? ((n + 63) >> 6) : 1;
-
-  uint32_t w_cnt = 0, w_exp = 0;
-  int w_kmin = INT_MAX;
-  u64 w_best = ~0ull;
+  ScoreAcc acc;
 
   for (int pb = (local * nwaves + wave) * PPW; pb < P; pb += nblk * nwaves * PPW) {
     const int p = pb + sub;
     const bool pact = p < P;
-    const int* par = a.beam_cur + (size_t)(pact ? p : 0) * S;
+    GL const int* par = gp(a.beam_cur) + (size_t)(pact ? p : 0) * S;
     const int pc = pact ? par[Ly.off_cmax] : 0;
-    int pos[2], mc[2], f[2], cm[2], nxv[2], nmc[2];
-    bool rdy[2];
-    int r = 0;
+    JobView v;
+    v.r = 0;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      rdy[h] = false; pos[h] = kFinishedPos; mc[h] = 0; f[h] = 0; cm[h] = 0; nxv[h] = 0; nmc[h] = 0;
+      v.rdy[h] = false; v.pos[h] = kFinishedPos; v.mc[h] = 0; v.f[h] = 0; v.cm[h] = 0; v.nx[h] = 0; v.nmc[h] = 0;
       const int j = h * 64 + gl;
       if (pact && h < NH && j < n) {
         const int w = par[Ly.off_jw + j];
         // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
         const int pred = (m == 1) ? (j > 0 ? par[Ly.off_jr + j - 1] : 0) : par[Ly.off_jr + j];
-        pos[h] = jw_pos(w);
-        if (pos[h] != kFinishedPos) {
-          rdy[h] = true;
+        v.pos[h] = jw_pos(w);
+        if (v.pos[h] != kFinishedPos) {
+          v.rdy[h] = true;
           const int nx = jw_next(w);
-          nxv[h] = nx;
-          mc[h] = jw_mc(w);
+          v.nx[h] = nx;
+          v.mc[h] = jw_mc(w);
           // Duration, the following op's machine and the machine's ready
           // time all depend on the job word only: one round of loads.
-          const int dur = a.ops[j * m + nx].y;
-          nmc[h] = next_machine(a.ops, j, m, nx);
-          const int st = max(par[Ly.off_mr + mc[h]], pred);
-          f[h] = st + dur;
-          cm[h] = max(pc, f[h]);
+          const int dur = ops[j * m + nx].y;
+          v.nmc[h] = next_machine(ops, j, m, nx);
+          const int st = max(par[Ly.off_mr + v.mc[h]], pred);
+          v.f[h] = st + dur;
+          v.cm[h] = max(pc, v.f[h]);
         }
       }
-      r += __popcll(__ballot(rdy[h]) & gmask);
+      v.r += __popcll(__ballot(v.rdy[h]) & gmask);
     }
-
-    int slot[2];
-    bool val[2];
-    int nvalid;
-    if (mode == 1) {
-      slot[0] = pos[0]; slot[1] = pos[1];
-      val[0] = rdy[0]; val[1] = rdy[1];
-      nvalid = r;
-    } else {
-      // Pivot: ready[hash % r] (gpu/v1/solver.cu:51-56); every lane reads its
-      // own group's source lane.
-      const uint32_t hsh = pivot_hash(a.lseed + (uint32_t)p);
-      const int tpos = r > 0 ? (int)(hsh % (uint32_t)r) : -1;
-      const u64 b0 = __ballot(rdy[0] && pos[0] == tpos) & gmask;
-      const u64 b1 = __ballot(rdy[1] && pos[1] == tpos) & gmask;
-      const int t0 = __shfl(mc[0], b0 ? __ffsll((long long)b0) - 1 : lane);
-      const int t1 = __shfl(mc[1], b1 ? __ffsll((long long)b1) - 1 : lane);
-      const int tmc = b0 ? t0 : (b1 ? t1 : -1);
-      val[0] = rdy[0] && mc[0] == tmc;
-      val[1] = rdy[1] && mc[1] == tmc;
-      // Slot = rank of the ready position among the pivot machine's ops:
-      // the group's candidate positions as a 128-bit mask.
-      u64 plo = 0, phi = 0;
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        if (val[h]) {
-          if (pos[h] < 64) plo |= 1ull << pos[h];
-          else phi |= 1ull << (pos[h] - 64);
-        }
-      }
-      for (int o = 1; o < G; o <<= 1) {
-        plo |= __shfl_xor(plo, o);
-        phi |= __shfl_xor(phi, o);
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int ps = pos[h];
-        slot[h] = ps < 64 ? __popcll(plo & ((1ull << ps) - 1ull))
-                          : __popcll(plo) + __popcll(phi & ((1ull << (ps - 64)) - 1ull));
-      }
-      nvalid = __popcll(plo) + __popcll(phi);
-    }
-    if (pact && gl == 0) w_exp += (uint32_t)nvalid;
-
+    int slot[2], nvalid;
     bool keep[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      val[h] = val[h] && slot[h] < B;  // always true for a consistent snapshot; keeps writes in bounds
-      keep[h] = val[h] && cm[h] < a.best_known;
-      uint32_t d = 0;
-      bool counted = false;
-      if (val[h]) {
-        const uint32_t x = (uint32_t)(p * B + slot[h]);
-        const int j = h * 64 + gl;
-        if (keep[h]) {
-          a.crec[x] = pack_rec(f[h], mc[h], j, nmc[h]);
-          if (last) {
-            const u64 c = ((u64)(uint32_t)cm[h] << 32) | x;
-            w_best = c < w_best ? c : w_best;
-          } else {
-            const float base = gpu_score_base(cm[h], ns);
-            const float sc = gpu_score_add_noise(base, noise_hash(a.lseed, (uint32_t)p, (uint32_t)slot[h]));
-            const uint32_t bits = __float_as_uint(sc);
-            a.keys[x] = ((u64)bits << 32) | x;
-            if (a.exact_digit) {
-              // Monotone in (score, child) when a lower cmax always scores
-              // lower (checked per instance on the host): cmax >= lo = cmin,
-              // and for one cmax the score grows with its ulp offset.
-              const uint32_t dc = min((uint32_t)(cm[h] - (int)lo), 256u);
-              const uint32_t off = min(bits - __float_as_uint(base), 7u);
-              d = min(8u * dc + off, (uint32_t)(kNB - 1));
-            } else {
-              d = digit1(bits, lo, s1);
-            }
-            a.kdig[x] = (uint16_t)d;
-            counted = true;
-            w_cnt++;
-            w_kmin = min(w_kmin, cm[h]);
-          }
-        } else if (!last) {
-          a.keys[x] = ~0ull;
-        }
-      }
-      lds_count(s_h, d, counted, lane);
-    }
-    if (!last && pact) {
-      for (int s = gl; s < B; s += G)
-        if (s >= nvalid) a.keys[(size_t)p * B + s] = ~0ull;
-    }
+    emit_children(dst, sl, p, pact, G, gl, gmask, lane, v, s_h, acc, slot, keep, nvalid);
 
-    if (single && !last) {  // G == 64: one parent per wave
-      const int order = (mode == 1) ? kOrderErase : kOrderSwap;
+    if (SINGLE && !sl.last) {  // G == 64: one parent per wave
+      const int order = (a.mode == 1) ? kOrderErase : kOrderSwap;
       int pw[kMaxWPL];
 #pragma unroll
       for (int i = 0; i < kMaxWPL; ++i) {
         const int o = lane + 64 * i;
         pw[i] = (pact && i < NW && o < S) ? par[o] : 0;
       }
+      const int r = v.r;
       int jlast = -1;
       if (order == kOrderSwap) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const u64 bm = __ballot(rdy[h] && pos[h] == r - 1);
+          const u64 bm = __ballot(v.rdy[h] && v.pos[h] == r - 1);
           if (bm) jlast = h * 64 + __ffsll((long long)bm) - 1;
         }
       }
@@ -458,15 +665,15 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
           bm &= bm - 1;
           ChildMod c;
           c.j = h * 64 + l;
-          c.mc = __shfl(mc[h], l);
-          c.f = __shfl(f[h], l);
-          c.cm = __shfl(cm[h], l);
-          c.spos = __shfl(pos[h], l);
+          c.mc = __shfl(v.mc[h], l);
+          c.f = __shfl(v.f[h], l);
+          c.cm = __shfl(v.cm[h], l);
+          c.spos = __shfl(v.pos[h], l);
           c.jlast = jlast;
           const int cs = __shfl(slot[h], l);
-          const int nx1 = __shfl(nxv[h], l) + 1;
-          c.newjw = nx1 < m ? jw_make(nx1, r - 1, __shfl(nmc[h], l)) : jw_make(nx1, kFinishedPos, 0);
-          int* out = a.cbuf + ((size_t)p * B + cs) * S;
+          const int nx1 = __shfl(v.nx[h], l) + 1;
+          c.newjw = nx1 < m ? jw_make(nx1, r - 1, __shfl(v.nmc[h], l)) : jw_make(nx1, kFinishedPos, 0);
+          GL int* out = gp(a.cbuf) + ((size_t)p * B + cs) * S;
           if (order == kOrderErase) {
 #pragma unroll
             for (int i = 0; i < kMaxWPL; ++i) {
@@ -484,30 +691,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
       }
     }
   }
-
-  // Block partials: kept count, expanded count, last-level best; histogram.
-  w_cnt = wave_sum(w_cnt);
-  w_exp = wave_sum(w_exp);
-  w_kmin = wave_min_i(w_kmin);
-  w_best = wave_min64(w_best);
-  if (lane == 0) { s_cnt[wave] = w_cnt; s_exp[wave] = w_exp; s_kmin[wave] = w_kmin; s_best[wave] = w_best; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    BlockStat b;
-    b.valid = 0; b.expanded = 0; b.min_cmax = INT_MAX; b.pad = 0; b.best = ~0ull;
-    for (int w = 0; w < nwaves; ++w) {
-      b.valid += s_cnt[w];
-      b.expanded += s_exp[w];
-      b.min_cmax = min(b.min_cmax, s_kmin[w]);
-      b.best = s_best[w] < b.best ? s_best[w] : b.best;
-    }
-    a.bstat[local] = b;
-  }
-  if (!last) {
-    uint32_t* hc = a.hist + (size_t)(local % kHistCopies) * kNB;
-    for (int i = threadIdx.x; i < kNB; i += blockDim.x)
-      if (s_h[i]) atomicAdd(&hc[i], s_h[i]);
-  }
+  finish_score_block(acc, s_h, a.bstat, gp(a.hist), local, sl.last);
 }
 
 // ---------------------------------------------------------------------------
@@ -736,6 +920,9 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ int s_last;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   SelState* sel = a.sel;
+  // The next level's histogram (read by the filter one level back) starts at
+  // zero before its score pass, which follows this kernel.
+  for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
 
   // The histogram loads go out first; they do not depend on the counts.
   uint32_t h0 = 0, h1 = 0;
@@ -858,27 +1045,30 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
 // ---------------------------------------------------------------------------
 // Word o of a snapshot whose words the wave holds as pw[i] = word lane + 64 i
 // (o wave-uniform).
+template <int NWM = kMaxWPL>
 __device__ __forceinline__ int wave_word(const int (&pw)[kMaxWPL], int o) {
   const int i = o >> 6;
   int v = 0;
 #pragma unroll
-  for (int k = 0; k < kMaxWPL; ++k) v = (k == i) ? pw[k] : v;
+  for (int k = 0; k < NWM; ++k) v = (k == i) ? pw[k] : v;
   return __shfl(v, o & 63);
 }
 
 // Builds survivor q of the next beam from its parent p and the child's record
-// (two-phase) and writes the trail entry.  One wave.  Returns the child's cmax.
-template <int ORDER>
+// (two-phase) and writes the trail entry.  One wave.  Returns the child's cmax;
+// nw[i] = the child's word lane + 64 i.
+// NWM: words per lane the caller supports (S <= 64 * NWM).
+template <int ORDER, int NWM = kMaxWPL>
 __device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout& Ly, int lane, int p, u64 rec,
-                                               int q) {
+                                               int q, int (&nw)[kMaxWPL]) {
   const int S = Ly.S, m = a.m;
   const int NW = (S + 63) >> 6;
-  const int* par = a.beam_cur + (size_t)p * S;
+  GL const int* par = gp(a.beam_cur) + (size_t)p * S;
   int pw[kMaxWPL];
   int r = 0;
   int jlast = -1;
 #pragma unroll
-  for (int i = 0; i < kMaxWPL; ++i) {
+  for (int i = 0; i < NWM; ++i) {
     const int o = lane + 64 * i;
     pw[i] = (i < NW && o < S) ? par[o] : 0;
     const bool isjw = (i < NW) && o >= Ly.off_jw && o < Ly.off_cmax;
@@ -886,7 +1076,7 @@ __device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout&
   }
   if (ORDER == kOrderSwap) {
 #pragma unroll
-    for (int i = 0; i < kMaxWPL; ++i) {
+    for (int i = 0; i < NWM; ++i) {
       const int o = lane + 64 * i;
       const bool isjw = (i < NW) && o >= Ly.off_jw && o < Ly.off_cmax;
       const u64 bm = __ballot(isjw && jw_pos(pw[i]) == r - 1);
@@ -894,42 +1084,48 @@ __device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout&
     }
   }
   const int j = rec_job(rec);
-  const int jwj = wave_word(pw, Ly.off_jw + j);
+  const int jwj = wave_word<NWM>(pw, Ly.off_jw + j);
   const int nx = jw_next(jwj);
   ChildMod c;
   c.j = j;
   c.mc = rec_machine(rec);
   c.f = rec_finish(rec);
-  c.cm = max(wave_word(pw, Ly.off_cmax), c.f);
+  c.cm = max(wave_word<NWM>(pw, Ly.off_cmax), c.f);
   c.spos = jw_pos(jwj);
   c.jlast = jlast;
   c.newjw = nx + 1 < m ? jw_make(nx + 1, r - 1, rec_next_machine(rec)) : jw_make(nx + 1, kFinishedPos, 0);
-  int* out = a.beam_next + (size_t)q * S;
+  GL int* out = gp(a.beam_next) + (size_t)q * S;
 #pragma unroll
   for (int i = 0; i < kMaxWPL; ++i) {
-    const int o = lane + 64 * i;
-    if (i < NW && o < S) out[o] = child_word<ORDER>(o, pw[i], c, Ly);
+    if (i < NWM && i < NW) {
+      const int o = lane + 64 * i;
+      nw[i] = child_word<ORDER>(o, pw[i], c, Ly);
+      if (o < S) out[o] = nw[i];
+    } else {
+      nw[i] = 0;
+    }
   }
-  if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | j;
+  if (lane == 0) gp(a.trail)[(size_t)a.level * a.K + q] = (p << 8) | j;
   return c.cm;
 }
 
 // Key of the r-th (0-based) tie in child-index order: tile c holds tcnt[c]
 // ties at ref_buf[c * kFilterThreads ...], tp[0..tiles] is the exclusive
 // prefix of those counts.  Wave-uniform.
-__device__ __forceinline__ u64 tie_key(const LevelArgs& a, const uint32_t* tp, int tiles, uint32_t r) {
+__device__ __forceinline__ u64 tie_key(GL const u64* tiles_keys, const uint32_t* tp, int tiles, uint32_t r) {
   int lo = 0, hi = tiles - 1;  // the last tile c with tp[c] <= r holds it
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (tp[mid] <= r) lo = mid;
     else hi = mid - 1;
   }
-  return a.ref_buf[(size_t)lo * kFilterThreads + (r - tp[lo])];
+  return tiles_keys[(size_t)lo * kFilterThreads + (r - tp[lo])];
 }
 
 // Number of the n keys of C below key, one wave; eight loads in flight per
 // step (C is LDS or global memory, known after inlining).
-__device__ __forceinline__ uint32_t count_below(const u64* C, int n, u64 key, int lane) {
+template <class P>
+__device__ __forceinline__ uint32_t count_below(P C, int n, u64 key, int lane) {
   uint32_t c = 0;
   int base = 0;
   for (; base + 512 <= n; base += 512) {
@@ -953,17 +1149,29 @@ __device__ __forceinline__ uint32_t count_below(const u64* C, int n, u64 key, in
 // index) order -- the order CUB's stable sort gives the reference).
 // Survivors u >= sf are the threshold bin's ties, which sort by child index
 // after every key of cand: survivor u is the (u - sf)-th tie, at rank u.
-__device__ void rank_body(const LevelArgs& a, int local, int nblk) {
+// MODE: 0 = the instance's mode at run time (separate score kernel), 1 =
+// ICH_DET, 2 = PMS_GPU (fused kernels are compiled per mode).  Fused: after
+// materialising a survivor, the wave writes its words to its LDS view and
+// scores its children for the next level.
+// SMALL (fused only): S <= 128 and n <= 64, so a snapshot is two words per
+// lane and a job one slot per lane.
+template <bool FUSED, int MODE, bool SMALL = false>
+__device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nblk) {
+  constexpr int NWM = SMALL ? 2 : kMaxWPL;
+  constexpr int NHM = SMALL ? 1 : 2;
   extern __shared__ u64 s_c[];
+  __shared__ uint32_t s_h[kNB];
   __shared__ int s_mc[kRankWaves];
   __shared__ uint32_t s_mh[kRankWaves], s_ml[kRankWaves], s_w[kRankWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
-  for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
+  if (FUSED)
+    for (int i = tid; i < kNB; i += blockDim.x) s_h[i] = 0;
   if (local == 0 && tid == 0) {
-    a.sel_next->sure_ctr = 0;
-    a.sel_next->tie_ctr = 0;
-    a.sel_next->done_ctr = 0;
+    GL SelState* sn = gp(a.sel_next);
+    sn->sure_ctr = 0;
+    sn->tie_ctr = 0;
+    sn->done_ctr = 0;
     if (a.sel_log) a.sel_log[a.level] = *a.sel;
   }
   // Everything this block stages is loaded before the selection state
@@ -973,23 +1181,28 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
   const int tiles_max = (int)(((uint32_t)a.K * (uint32_t)a.B + kFilterThreads - 1) / kFilterThreads);
   const int per = (tiles_max + (int)blockDim.x - 1) / (int)blockDim.x;
   const int t0 = min(tiles_max, tid * per);
+  GL const u64* cand = gp(a.cand);
+  GL const uint32_t* tcnt = gp(a.tcnt);
+  GL const SelState* sel = gp(a.sel);
   u64 cv[kCandRun];
   uint32_t tc[kTileRun];
   if (in_lds) {
 #pragma unroll
     for (int k = 0; k < kCandRun; ++k) {
       const int i = tid + k * (int)blockDim.x;
-      cv[k] = i < a.K ? a.cand[i] : ~0ull;
+      cv[k] = i < a.K ? cand[i] : ~0ull;
     }
   }
 #pragma unroll
-  for (int k = 0; k < kTileRun; ++k) tc[k] = (k < per && t0 + k < tiles_max) ? a.tcnt[t0 + k] : 0u;
-  const int Kp = (int)a.sel->Kp;
-  const int sf = min((int)a.sel->sorted_from, Kp);
+  for (int k = 0; k < kTileRun; ++k) tc[k] = (k < per && t0 + k < tiles_max) ? tcnt[t0 + k] : 0u;
+  const int Kp = (int)sel->Kp;
+  const int sf = min((int)sel->sorted_from, Kp);
   const bool ties = sf < Kp;
   uint32_t* s_tp = reinterpret_cast<uint32_t*>(s_c + (in_lds ? a.K : 0));
-  const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
+  const uint32_t N = (uint32_t)gp(a.pcount)[a.level] * (uint32_t)a.B;
   const int tiles = (int)((N + kFilterThreads - 1) / kFilterThreads);
+  // Fused: per-wave snapshot views after the tile prefix (16-byte aligned).
+  int* s_view = reinterpret_cast<int*>(s_tp + ((tiles_max + 1 + 3) & ~3));
   if (in_lds) {
 #pragma unroll
     for (int k = 0; k < kCandRun; ++k) {
@@ -1019,11 +1232,26 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
   __syncthreads();
   const Layout Ly = make_layout(a.n, a.m);
   const int S = Ly.S;
+  const int NW = (S + 63) >> 6;
+  // Fused: the next level's scoring parameters (exact digit: lo from this
+  // level's filter).
+  ScoreLevel sln;
+  sln.lo = FUSED ? gp(a.sel_next)->lo : 0u;
+  sln.s1 = kLevel0Shift;
+  sln.ns = a.level + 2;
+  sln.last = (a.level + 1 == a.L - 1);
+  ScoreDst dn = score_dst(a);  // the next level's keys, digits, records
+  dn.crec = gp(crec_next);
+  dn.lseed = a.lseed + 1u;
+  if (MODE != 0) dn.mode = MODE;
+  GL const int2* ops = gp(a.ops);
+  GL const u64* crec = gp(a.crec);
+  ScoreAcc acc;
   int w_min = INT_MAX;
   uint32_t w_max = 0, w_minh = 0xFFFFFFFFu;
   const int nwaves = kRankWaves;
-  const bool single = a.cbuf != nullptr;
-  const int order = (a.mode == 1) ? kOrderErase : kOrderSwap;
+  const bool single = !FUSED && a.cbuf != nullptr;
+  const int order = MODE == 1 ? kOrderErase : (MODE == 2 ? kOrderSwap : ((a.mode == 1) ? kOrderErase : kOrderSwap));
   for (int u = local * nwaves + wave; u < Kp; u += nblk * nwaves) {
     uint32_t cnt = (uint32_t)u;
     u64 key;
@@ -1032,27 +1260,44 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
         key = s_c[u];
         cnt = count_below(s_c, sf, key, lane);
       } else {
-        key = a.cand[u];
-        cnt = count_below(a.cand, sf, key, lane);
+        key = cand[u];
+        cnt = count_below(cand, sf, key, lane);
       }
     } else {
-      key = tie_key(a, s_tp, tiles, (uint32_t)(u - sf));
+      key = tie_key(gp(a.ref_buf), s_tp, tiles, (uint32_t)(u - sf));
     }
     const int q = (int)cnt;
     const uint32_t x = (uint32_t)key;
     const uint32_t hi = (uint32_t)(key >> 32);
     const int p = (int)(x / (uint32_t)a.B);
-    const u64 rec = a.crec[x];
+    const u64 rec = crec[x];
     int cm;
     if (single) {
-      const int* src = a.cbuf + (size_t)x * S;
-      int* dst = a.beam_next + (size_t)q * S;
+      GL const int* src = gp(a.cbuf) + (size_t)x * S;
+      GL int* dst = gp(a.beam_next) + (size_t)q * S;
       for (int o = lane; o < S; o += 64) dst[o] = src[o];
       cm = src[Ly.off_cmax];
-      if (lane == 0) a.trail[(size_t)a.level * a.K + q] = (p << 8) | rec_job(rec);
+      if (lane == 0) gp(a.trail)[(size_t)a.level * a.K + q] = (p << 8) | rec_job(rec);
     } else {
-      cm = (order == kOrderErase) ? materialize_rec<kOrderErase>(a, Ly, lane, p, rec, q)
-                                  : materialize_rec<kOrderSwap>(a, Ly, lane, p, rec, q);
+      int nw[kMaxWPL];
+      cm = (order == kOrderErase) ? materialize_rec<kOrderErase, NWM>(a, Ly, lane, p, rec, q, nw)
+                                  : materialize_rec<kOrderSwap, NWM>(a, Ly, lane, p, rec, q, nw);
+      if (FUSED) {
+        // Score the survivor's children for the next level from its words.
+        int* vw = s_view + wave * S;
+#pragma unroll
+        for (int i = 0; i < NWM; ++i) {
+          const int o = lane + 64 * i;
+          if (i < NW && o < S) vw[o] = nw[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        JobView v = view_from_lds<NHM>(ops, Ly, vw, cm, lane);
+        int slot[2], nvalid;
+        bool keep[2];
+        emit_children<NHM>(dn, sln, q, true, 64, lane, ~0ull, lane, v, s_h, acc, slot, keep, nvalid);
+      }
     }
     w_min = min(w_min, cm);
     w_max = max(w_max, hi);
@@ -1072,6 +1317,7 @@ __device__ void rank_body(const LevelArgs& a, int local, int nblk) {
     }
     a.rstat[local] = rs;
   }
+  if (FUSED) finish_score_block(acc, s_h, a.bstat, gp(a.hist_next), local, sln.last);
 }
 
 // ---------------------------------------------------------------------------
@@ -1119,7 +1365,8 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
   for (int q = blockIdx.x * 4 + wave; q < count; q += gridDim.x * 4) {
     const int x = a.ex_order[q];
     const int p = x / a.B;
-    (void)materialize_rec<kOrderErase>(a, Ly, lane, p, a.crec[x], q);
+    int nw[kMaxWPL];
+    (void)materialize_rec<kOrderErase>(a, Ly, lane, p, a.crec[x], q, nw);
   }
 }
 
@@ -1200,38 +1447,75 @@ __device__ __forceinline__ LevelArgs resolve(const InstDesc& d, int level) {
   a.sel = d.sel[level & 1];
   a.sel_prev = d.sel[(level + 1) & 1];
   a.sel_next = d.sel[(level + 1) & 1];
+  a.crec = d.crec[level & 1];
   return a;
 }
 
+// Root of every instance mapped here; in the fused form, wave 0 also scores
+// the root's children (level 0), which the rank kernel does for every later
+// level.
 __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* map) {
+  __shared__ uint32_t s_h[kNB];
+  __shared__ int s_bound;
   InstDesc& d = desc[block_ref(map).x];
-  const LevelArgs& b = d.base;
-  const Layout Ly = make_layout(b.n, b.m);
+  const Layout Ly = make_layout(d.base.n, d.base.m);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int i = threadIdx.x; i < Ly.S; i += blockDim.x) {
     int v = 0;
     const int j = i - Ly.off_jw;
-    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, j, b.ops[j * b.m].x);  // ready = [0..n-1]
+    if (i >= Ly.off_jw && i < Ly.off_cmax) v = jw_make(0, j, d.base.ops[j * d.base.m].x);  // ready = [0..n-1]
     d.beam[0][i] = v;
   }
   for (int i = threadIdx.x; i < kNB * kHistCopies; i += blockDim.x) {
     d.hist[0][i] = 0;
     d.hist[1][i] = 0;
   }
+  for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
   if (threadIdx.x == 0) {
     // Multi-seed passes prune with the shared bound as it stands when the
     // pass starts, so every beam of a pass sees the same bound.
     if (d.shared_bound) d.base.best_known = *d.shared_bound;
+    s_bound = d.base.best_known;
     SelState z = {};
+    z.s1 = kLevel0Shift;
+    z.exact = d.base.exact_digit ? 1u : 0u;
     *d.sel[0] = z;
     *d.sel[1] = z;
-    b.pcount[0] = 1;
-    *b.expanded = 0ull;
-    b.result[0] = b.best_known; b.result[1] = 0; b.result[2] = 0;
+    d.base.pcount[0] = 1;
+    *d.base.expanded = 0ull;
+    d.base.result[0] = d.base.best_known; d.base.result[1] = 0; d.base.result[2] = 0;
     d.lvl_score = 0;
     d.lvl_filter = 0;
-    d.lvl_rank = -1;  // the first score launch advances it to 0
+    d.lvl_rank = -1;  // the first score (or, fused, filter) launch advances it to 0
     d.bt_done = 0;
   }
+  __syncthreads();
+  if (!d.fused) return;
+  LevelArgs a = resolve(d, 0);
+  a.best_known = s_bound;
+  ScoreLevel sl;
+  sl.lo = 0;  // the root's cmax
+  sl.s1 = kLevel0Shift;
+  sl.ns = 1;
+  sl.last = (a.L == 1);
+  ScoreAcc acc;
+  if (wave == 0) {
+    const int NW = (Ly.S + 63) >> 6;
+    int w[kMaxWPL];
+#pragma unroll
+    for (int i = 0; i < kMaxWPL; ++i) {
+      const int o = lane + 64 * i;
+      const int j = o - Ly.off_jw;
+      w[i] = (i < NW && o >= Ly.off_jw && o < Ly.off_cmax) ? jw_make(0, j, a.ops[j * a.m].x) : 0;
+    }
+    JobView v = view_from_words(gp(a.ops), Ly, w, NW, 0, lane);
+    int slot[2], nvalid;
+    bool keep[2];
+    emit_children(score_dst(a), sl, 0, true, 64, lane, ~0ull, lane, v, s_h, acc, slot, keep, nvalid);
+  }
+  finish_score_block(acc, s_h, a.bstat, gp(a.hist), 0, sl.last);
+  const BlockStat empty = {0u, 0u, INT_MAX, 0u, ~0ull};
+  for (int i = 1 + (int)threadIdx.x; i < a.nblk_score; i += blockDim.x) a.bstat[i] = empty;
 }
 
 // Advances a level counter (block 0 of an instance, after its reader kernel
@@ -1266,20 +1550,37 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, con
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_filter;
-  advance(d.lvl_score, r.y, d.base.L);
+  if (d.fused) advance(d.lvl_rank, r.y, d.base.L);
+  else advance(d.lvl_score, r.y, d.base.L);
   if (level >= d.base.L - 1) return;  // the last level only records the best child
   const LevelArgs a = resolve(d, level);
   filter_body(a, r.y, d.nblk_filter);
 }
 
+// Rank + materialise; instances of the other form return at once (a batch
+// mixing both forms launches both kernels over the same map).
 __global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
+  if (d.fused) return;
   const int level = d.lvl_rank;
   advance(d.lvl_filter, r.y, d.base.L);
   if (level >= d.base.L - 1) return;
   const LevelArgs a = resolve(d, level);
-  rank_body(a, r.y, d.nblk_rank);
+  rank_body<false, 0>(a, nullptr, r.y, d.nblk_rank);
+}
+
+// Rank + materialise + score of the next level (fused form), per mode.
+template <int MODE, bool SMALL>
+__global__ __launch_bounds__(64 * kRankWaves) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
+  const BlockRef r = block_ref(map);
+  InstDesc& d = desc[r.x];
+  if (!d.fused || d.base.mode != MODE || (d.base.S <= 128 && d.base.n <= 64) != SMALL) return;
+  const int level = d.lvl_rank;
+  advance(d.lvl_filter, r.y, d.base.L);
+  if (level >= d.base.L - 1) return;
+  const LevelArgs a = resolve(d, level);
+  rank_body<true, MODE, SMALL>(a, d.crec[(level + 1) & 1], r.y, d.nblk_rank);
 }
 
 __global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map) {
@@ -1330,9 +1631,17 @@ hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, siz
   return hipGetLastError();
 }
 
-hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
-                            const EventPair* ev) {
-  launch(k_rank_b, dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
+                            hipStream_t s, const EventPair* ev) {
+  void (*k)(InstDesc*, const BlockRef*) = k_rank_b;
+  switch (variant) {
+    case kRankFusedIch: k = k_rank_fused_b<1, false>; break;
+    case kRankFusedPms: k = k_rank_fused_b<2, false>; break;
+    case kRankFusedIchSmall: k = k_rank_fused_b<1, true>; break;
+    case kRankFusedPmsSmall: k = k_rank_fused_b<2, true>; break;
+    default: break;
+  }
+  launch(k, dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
   return hipGetLastError();
 }
 

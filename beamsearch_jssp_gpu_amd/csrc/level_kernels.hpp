@@ -27,6 +27,9 @@ constexpr int kRankWaves = 16;
 // Survivors ranked in LDS (8 bytes each) up to this beam width; wider beams
 // rank from global memory.
 constexpr int kRankLdsMax = 16384;
+// Dynamic LDS the fused rank kernel may use (its static LDS is ~9 KB of 160 KB);
+// solves that would need more use the separate score kernel.
+constexpr size_t kRankFusedLdsMax = 144 * 1024;
 // LDS bitmap over child indices for the threshold cut: up to 2^20 children.
 constexpr uint32_t kBitmapWordsMax = 32768;
 
@@ -63,7 +66,8 @@ struct LevelArgs {
   int* beam_next;            // K x S
   int* cbuf;                 // single-phase children, (K*B) x S
   unsigned long long* keys;  // K*B (also the refinement scratch)
-  unsigned long long* crec;  // K*B: kept child's record, finish | machine << 32 | job << 40
+  unsigned long long* crec;  // K*B: kept child's record, finish | machine << 32 | job << 40 | next
+                             // machine << 48 (two buffers by level parity)
   unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
   unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut / the ties of each filter tile
   uint16_t* kdig;            // K*B: first selection digit of each key
@@ -108,6 +112,7 @@ struct InstDesc {
   int* beam[2];
   uint32_t* hist[2];
   SelState* sel[2];
+  unsigned long long* crec[2];
   int nblk_score, nblk_filter, nblk_rank;
   // Level each step kernel runs next.  Each counter is advanced by block 0
   // of the kernel that follows its reader in the step (score -> filter ->
@@ -115,6 +120,9 @@ struct InstDesc {
   // counter change while its kernel runs; all saturate at L.
   int lvl_score, lvl_filter, lvl_rank;
   int bt_done;           // sequence rebuilt
+  int fused;             // the next level is scored inside the rank kernel (no score launch):
+                         // the filter advances lvl_rank, base.nblk_score = nblk_rank partials
+  int pad2;
   int* shared_bound;     // multi-seed: device best bound read at init, lowered at the end
 };
 
@@ -135,8 +143,20 @@ hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool
                              const EventPair* ev = nullptr);
 hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
                               const EventPair* ev = nullptr);
-hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
-                            const EventPair* ev = nullptr);
+// Rank kernel variants: the separate-score form, and the fused form per mode
+// (ICH_DET, PMS_GPU) and size (small: S <= 128 words and n <= 64 jobs).  Each
+// returns at once for instances of another variant, so a batch mixing them
+// launches each variant it holds.
+enum RankVariant : int {
+  kRankUnfused = 0,
+  kRankFusedIch = 1,
+  kRankFusedPms = 2,
+  kRankFusedIchSmall = 3,
+  kRankFusedPmsSmall = 4,
+  kRankVariants = 5
+};
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
+                            hipStream_t s, const EventPair* ev = nullptr);
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
 
 // Exact (host-replay) modes, driven level by level from the host.

@@ -16,7 +16,7 @@ struct Work {
   int* beam[2];
   int* cbuf;
   unsigned long long* keys;
-  unsigned long long* crec;
+  unsigned long long* crec[2];
   unsigned long long* cand;
   unsigned long long* ref_buf;
   uint16_t* kdig;
@@ -44,13 +44,21 @@ struct Shape {
   int n, m, S, B, K, L;
   bool single, exact;
   int G;                       // lanes per parent in the score kernel
+  bool fused;                  // the rank kernel scores the next level (no score kernel)
   int nblk_score, nblk_filter, nblk_rank;
 };
 
 // Lanes per parent: the smallest group that holds every job (64 above 32).
 int default_lanes(int n_jobs);
-// G = 0 picks default_lanes; single-phase always uses 64.
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int G);
+// Score shape code of the fused form: each level's scoring runs inside the
+// previous level's rank kernel, one wave per parent.
+constexpr int kLanesFused = 1;
+// Whether the fused form applies: two-phase, a device mode, the exact digit.
+bool fused_eligible(const jssp_instance* in, const jssp_params* p);
+// lanes: 0 = automatic (fused when eligible, else default_lanes), kLanesFused,
+// or 16 / 32 / 64 lanes per parent in a separate score kernel (single-phase
+// always uses a 64-lane score kernel).
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes);
 // Bytes of the workspace (base == nullptr) or carve it at base.
 size_t carve(const Shape& s, char* base, Work& w);
 LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p);
@@ -63,6 +71,8 @@ void append_maps(const Shape& s, int inst, std::vector<BlockRef>& score, std::ve
 // Dynamic LDS of the filter (threshold bitmap) and rank (survivor keys) kernels.
 size_t filter_lds_bytes(const Shape& s, const jssp_params* p);
 size_t rank_lds_bytes(const Shape& s);
+// Rank kernel variant of a solve (RankVariant).
+int rank_variant(const Shape& s, const jssp_params* p);
 // Writes (machine, duration) pairs to the staging buffer and copies them to d_ops.
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops);
 

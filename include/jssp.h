@@ -72,6 +72,8 @@ typedef enum jssp_mode {
 #define JSSP_FLAG_GLOBAL_RAND 0x1u  /* exact modes: do not srand(seed); continue the process-global rand() stream */
 #define JSSP_FLAG_RADIX_CUT 0x2u    /* diagnostics: cut the selection threshold bin with the generic radix select only */
 #define JSSP_FLAG_GENERIC_DIGIT 0x4u /* diagnostics: use the generic score-bit selection digit only */
+#define JSSP_FLAG_UNFUSED 0x8u       /* diagnostics: score each level in its own kernel, never inside the
+                                       previous level's rank kernel (same results) */
 
 typedef struct jssp_instance {
   int32_t n_jobs;
@@ -134,13 +136,17 @@ int jssp_solve_batch(jssp_engine* engine, int32_t count, const jssp_instance* in
 int jssp_solve_batch_ex(jssp_engine* engine, int32_t count, const jssp_instance* insts,
                         const jssp_params* params, jssp_result* results, int32_t use_graph);
 
-/* Lanes-per-parent autotuner of the score kernel (16, 32 or 64 lanes per
- * parent; tile = 16 waves).  When on (the default), a batch first times a few
- * levels of each (n, m, K) class it has not seen with every valid shape and
- * keeps the fastest; table_path (or NULL) persists the table as text lines
- * "n m log2K lanes".  The shape never changes results. */
+/* Score-shape autotuner: per (n, m, K) class, either a separate score kernel
+ * with 16, 32 or 64 lanes per parent (tile = 16 waves), or (lanes = 1) the
+ * fused form where the rank kernel scores the next level, one wave per
+ * parent (two-phase device modes with the exact selection digit only).  When
+ * on (the default), a batch first times a few levels of each class it has not
+ * seen with every valid shape and keeps the fastest; table_path (or NULL)
+ * persists the table as text lines "n m log2K lanes".  Without a tuned entry
+ * a solve uses the fused form where it applies.  The shape never changes
+ * results. */
 int jssp_engine_set_autotune(jssp_engine* engine, int32_t on, const char* table_path);
-/* Tuned lanes per parent for a class, 0 if not tuned yet. */
+/* Tuned shape of a class (1 = fused, else lanes per parent), 0 if not tuned yet. */
 int jssp_engine_tuned_lanes(const jssp_engine* engine, int32_t n_jobs, int32_t n_machines, int32_t beam_width);
 
 /* Per-kernel timing accumulated over the solves run since profiling was

@@ -46,23 +46,28 @@ def test_batch_equals_single_solves(engine):
         np.testing.assert_array_equal(g.sequence, w["sequence"])
 
 
+def _shape(lanes, n):
+    return lanes if (lanes == bs.LANES_FUSED or lanes >= n) else 64
+
+
 def test_batch_lane_shapes_do_not_change_results(engine, tmp_path):
-    """Each lanes-per-parent shape (16 / 32 / 64) gives the same beams."""
+    """Each score shape (fused into the rank kernel, or 16 / 32 / 64 lanes per
+    parent in a separate kernel) gives the same beams."""
     jobs = _mixed(2, 9, ns=(10, 15, 20, 30))
     want = [engine.solve(_b(i), **p) for i, p in jobs]
-    for lanes in (16, 32, 64):
+    for lanes in (bs.LANES_FUSED, 16, 32, 64):
         table = tmp_path / f"tune{lanes}.txt"
         with open(table, "w") as f:
             for inst, p in jobs:
                 kb = int(np.ceil(np.log2(p["beam_width"])))
-                f.write(f"{inst.n} {inst.m} {kb} {lanes if lanes >= inst.n else 64}\n")
+                f.write(f"{inst.n} {inst.m} {kb} {_shape(lanes, inst.n)}\n")
         eng = bs.Engine(0)
         eng.set_autotune(True, str(table))
         got = eng.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
         for g, s in zip(got, want):
             _same(g, s)
         for inst, p in jobs:
-            assert eng.tuned_lanes(inst.n, inst.m, p["beam_width"]) in (lanes if lanes >= inst.n else 64,)
+            assert eng.tuned_lanes(inst.n, inst.m, p["beam_width"]) == _shape(lanes, inst.n)
         eng.close()
 
 

@@ -55,14 +55,18 @@ def test_det_modes_20x20_k1024(engine, mode, phases):
     _check_det(engine, inst, 1024, mode, phases, seed=12345)
 
 
-SELECT_PATHS = [0, bs.FLAG_GENERIC_DIGIT, bs.FLAG_GENERIC_DIGIT | bs.FLAG_RADIX_CUT]
+# 0: exact digit, tile-listed ties, next level scored inside the rank kernel;
+# UNFUSED: the same with a separate score kernel; GENERIC_DIGIT: score-bit digit
+# and the exact threshold cut; + RADIX_CUT: the radix-select cut.
+SELECT_PATHS = [0, bs.FLAG_UNFUSED, bs.FLAG_GENERIC_DIGIT, bs.FLAG_GENERIC_DIGIT | bs.FLAG_RADIX_CUT]
 
 
 @pytest.mark.parametrize("flags", SELECT_PATHS)
 @pytest.mark.parametrize("mode", [bs.MODE_ICH_DET, bs.MODE_PMS_GPU])
 def test_det_selection_paths(engine, mode, flags):
-    """Every selection path (exact digit + index-bitmap cut; generic digit +
-    exact score histograms; generic digit + radix cut) gives the oracle's beams."""
+    """Every selection / scoring path (exact digit with the fused or the
+    separate score kernel; generic digit + exact score histograms; generic
+    digit + radix cut) gives the oracle's beams."""
     for s in (5, 6):
         inst = splitmix_instance(s, 30, 10, 1, 99)
         _check_det(engine, inst, 512, mode, 2, seed=s, flags=flags)
