@@ -212,11 +212,6 @@ __device__ __forceinline__ int child_word(int o, int pw, const ChildMod& c, cons
 
 // Words per lane of a snapshot (S <= 2*128 + 255 + 1 = 512 -> 8).
 constexpr int kMaxWPL = 8;
-// Filter tiles per rank thread in the tie-count prefix (K*B / kFilterThreads
-// tiles <= 65536 * 128 / 1024 = 8192 = 8 per thread of 1024).
-constexpr int kTileRun = 8;
-// Survivor keys per rank thread staged into LDS (kRankLdsMax / 1024).
-constexpr int kCandRun = kRankLdsMax / 1024;
 
 // ---------------------------------------------------------------------------
 __global__ void k_init(LevelArgs a) {
@@ -912,7 +907,7 @@ __device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bu
 __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_ex[kNB];
   __shared__ uint32_t s_w[16];
-  __shared__ uint32_t s_v[16], s_e[16];
+  __shared__ uint32_t s_e[16];
   __shared__ int s_km[16];
   __shared__ uint32_t s_T;
   __shared__ uint32_t s_off[2][16];
@@ -924,31 +919,42 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   // zero before its score pass, which follows this kernel.
   for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
 
-  // The histogram loads go out first; they do not depend on the counts.
+  // Every load of the prologue goes out at once: the histogram copies, one
+  // score-block partial per thread (nblk_score <= 512), the level's parents.
   uint32_t h0 = 0, h1 = 0;
 #pragma unroll
   for (int c = 0; c < kHistCopies; ++c) {
-    const uint2 hv = *reinterpret_cast<const uint2*>(a.hist + (size_t)c * kNB + 2 * tid);
-    h0 += hv.x;
-    h1 += hv.y;
+    const u64 hv = *reinterpret_cast<GL const u64*>(gp(a.hist) + (size_t)c * kNB + 2 * tid);
+    h0 += (uint32_t)hv;
+    h1 += (uint32_t)(hv >> 32);
   }
-  uint32_t v = 0, ex = 0;
+  uint32_t ex = 0;
   int km = INT_MAX;
-  for (int i = tid; i < a.nblk_score; i += blockDim.x) {
-    v += a.bstat[i].valid;
-    ex += a.bstat[i].expanded;
-    km = min(km, a.bstat[i].min_cmax);
+  if (tid < a.nblk_score) {
+    ex = gp(a.bstat)[tid].expanded;
+    km = gp(a.bstat)[tid].min_cmax;
   }
-  v = wave_sum(v);
+  const uint32_t N = (uint32_t)gp(a.pcount)[a.level] * (uint32_t)a.B;
+  // Exclusive scan of the histogram (1024 threads x 2 bins); its total is the
+  // number of kept children.
+  const uint32_t sum = h0 + h1;
+  const uint32_t inc = wave_incl_scan(sum, lane);
   ex = wave_sum(ex);
   km = wave_min_i(km);
-  if (lane == 0) { s_v[wave] = v; s_e[wave] = ex; s_km[wave] = km; }
+  if (lane == 63) s_w[wave] = inc;
+  if (lane == 0) { s_e[wave] = ex; s_km[wave] = km; }
   if (tid == 0) s_T = kNB - 1;
   __syncthreads();
-  v = 0; ex = 0; km = INT_MAX;
-  for (int w = 0; w < 16; ++w) { v += s_v[w]; ex += s_e[w]; km = min(km, s_km[w]); }
+  uint32_t wofs = 0, v = 0;
+  for (int w = 0; w < 16; ++w) {
+    wofs += (w < wave) ? s_w[w] : 0u;
+    v += s_w[w];
+  }
   const uint32_t Kp = min(v, (uint32_t)a.K);
   if (local == 0 && tid == 0) {
+    ex = 0;
+    km = INT_MAX;
+    for (int w = 0; w < 16; ++w) { ex += s_e[w]; km = min(km, s_km[w]); }
     // Lower bound of the next level's parents' cmax (they are kept children).
     if (a.exact_digit && km != INT_MAX) a.sel_next->lo = (uint32_t)km;
     sel->V = v;
@@ -958,16 +964,8 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     a.lvl_exp[a.level] = ex;
   }
   if (Kp == 0) return;
-  const uint32_t s1 = sel->s1;
   const bool exact = a.exact_digit != 0;
-
-  // Exclusive scan of the histogram: 1024 threads x 2 bins.
-  const uint32_t sum = h0 + h1;
-  const uint32_t inc = wave_incl_scan(sum, lane);
-  if (lane == 63) s_w[wave] = inc;
-  __syncthreads();
-  uint32_t wofs = 0;
-  for (int w = 0; w < wave; ++w) wofs += s_w[w];
+  const uint32_t s1 = exact ? 0u : sel->s1;
   const uint32_t ex0 = wofs + inc - sum;
   s_ex[2 * tid] = ex0;
   s_ex[2 * tid + 1] = ex0 + h0;
@@ -986,13 +984,16 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     sel->sorted_from = single ? cum_lt : Kp;
   }
 
-  const uint32_t N = (uint32_t)a.pcount[a.level] * (uint32_t)a.B;
+  GL const u64* keys = gp(a.keys);
+  GL const uint16_t* kdig = gp(a.kdig);
+  GL u64* cand = gp(a.cand);
+  GL u64* tiles_keys = gp(a.ref_buf);
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
     // Both loads at once: the digit of an invalid slot is never read.
-    const u64 key = inb ? a.keys[x] : ~0ull;
-    const uint32_t dg = inb ? (uint32_t)a.kdig[x] : (uint32_t)kNB;
+    const u64 key = inb ? keys[x] : ~0ull;
+    const uint32_t dg = inb ? (uint32_t)kdig[x] : (uint32_t)kNB;
     const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;
     const bool tie = d == T1;
@@ -1010,17 +1011,17 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
         ts += cs; tt += ct;
       }
       s_base[0] = ts ? atomicAdd(&sel->sure_ctr, ts) : 0u;
-      if (single) a.tcnt[base / blockDim.x] = tt;
+      if (single) gp(a.tcnt)[base / blockDim.x] = tt;
       else s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
     }
     __syncthreads();
-    if (sure) a.cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
+    if (sure) cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
     if (tie) {
       // Single-valued bin: the tile's ties in child-index order at the
       // tile's own offset (no atomics); otherwise compacted after the sure keys.
       const uint32_t o = s_off[1][wave] + lanes_below(bt, lane);
-      if (single) a.ref_buf[base + o] = key;
-      else a.cand[s_base[1] + o] = key;
+      if (single) tiles_keys[base + o] = key;
+      else cand[s_base[1] + o] = key;
     }
     __syncthreads();
   }
@@ -1174,58 +1175,44 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     sn->done_ctr = 0;
     if (a.sel_log) a.sel_log[a.level] = *a.sel;
   }
-  // Everything this block stages is loaded before the selection state
-  // arrives: the first K candidate keys (cand holds K*B) and the per-tile
-  // tie counts of the widest level (runs of kTileRun per thread).
+  // The selection state, this thread's first tie count and the survivor keys
+  // are loaded together; the keys go to LDS whatever the survivor count
+  // (cand holds K*B entries, and entries past the survivors are never read).
   const bool in_lds = a.K <= kRankLdsMax;
   const int tiles_max = (int)(((uint32_t)a.K * (uint32_t)a.B + kFilterThreads - 1) / kFilterThreads);
-  const int per = (tiles_max + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int per = (tiles_max + (int)blockDim.x - 1) / (int)blockDim.x;  // tiles per thread (<= 8)
   const int t0 = min(tiles_max, tid * per);
   GL const u64* cand = gp(a.cand);
   GL const uint32_t* tcnt = gp(a.tcnt);
   GL const SelState* sel = gp(a.sel);
-  u64 cv[kCandRun];
-  uint32_t tc[kTileRun];
+  const uint32_t kp_raw = sel->Kp, sf_raw = sel->sorted_from;
+  const uint32_t N = (uint32_t)gp(a.pcount)[a.level] * (uint32_t)a.B;
+  const uint32_t tc0 = t0 < tiles_max ? tcnt[t0] : 0u;
   if (in_lds) {
-#pragma unroll
-    for (int k = 0; k < kCandRun; ++k) {
-      const int i = tid + k * (int)blockDim.x;
-      cv[k] = i < a.K ? cand[i] : ~0ull;
-    }
+#pragma unroll 4
+    for (int i = tid; i < a.K; i += blockDim.x) s_c[i] = cand[i];
   }
-#pragma unroll
-  for (int k = 0; k < kTileRun; ++k) tc[k] = (k < per && t0 + k < tiles_max) ? tcnt[t0 + k] : 0u;
-  const int Kp = (int)sel->Kp;
-  const int sf = min((int)sel->sorted_from, Kp);
+  const int Kp = (int)kp_raw;
+  const int sf = min((int)sf_raw, Kp);
   const bool ties = sf < Kp;
   uint32_t* s_tp = reinterpret_cast<uint32_t*>(s_c + (in_lds ? a.K : 0));
-  const uint32_t N = (uint32_t)gp(a.pcount)[a.level] * (uint32_t)a.B;
   const int tiles = (int)((N + kFilterThreads - 1) / kFilterThreads);
   // Fused: per-wave snapshot views after the tile prefix (16-byte aligned).
   int* s_view = reinterpret_cast<int*>(s_tp + ((tiles_max + 1 + 3) & ~3));
-  if (in_lds) {
-#pragma unroll
-    for (int k = 0; k < kCandRun; ++k) {
-      const int i = tid + k * (int)blockDim.x;
-      if (i < sf) s_c[i] = cv[k];
-    }
-  }
-  // Tiles past this level's count hold stale counts.
-#pragma unroll
-  for (int k = 0; k < kTileRun; ++k) tc[k] = (t0 + k < tiles) ? tc[k] : 0u;
   if (ties) {
-    uint32_t loc = 0;
-#pragma unroll
-    for (int k = 0; k < kTileRun; ++k) loc += tc[k];
+    // Exclusive prefix of the per-tile tie counts (tiles past this level's
+    // count hold stale counts and are skipped).
+    const int t1 = min(tiles, t0 + per);
+    uint32_t loc = t0 < tiles ? tc0 : 0u;
+    for (int t = t0 + 1; t < t1; ++t) loc += tcnt[t];
     const uint32_t inc = wave_incl_scan(loc, lane);
     if (lane == 63) s_w[wave] = inc;
     __syncthreads();
     uint32_t run = inc - loc;
     for (int w = 0; w < wave; ++w) run += s_w[w];
-#pragma unroll
-    for (int k = 0; k < kTileRun; ++k) {
-      if (k < per && t0 + k < tiles) s_tp[t0 + k] = run;
-      run += tc[k];
+    for (int t = t0; t < t1; ++t) {
+      s_tp[t] = run;
+      run += (t == t0) ? tc0 : tcnt[t];
     }
     if (tid == (int)blockDim.x - 1) s_tp[tiles] = run;
   }
