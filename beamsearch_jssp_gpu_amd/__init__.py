@@ -25,7 +25,7 @@ __all__ = [
     "JSSPInstance", "Result", "Engine", "JsspError", "load_library", "library_path",
     "generate_instance", "parse_instance", "resolve_instance_file", "validate_sequence",
     "lower_bound", "beam_search_ich", "beam_search_pms", "run_gpu_beam_search_v1",
-    "run_gpu_beam_search_v2", "default_engine", "Server",
+    "run_gpu_beam_search_v2", "default_engine", "Server", "rand_stream",
 ]
 
 NO_MAKESPAN = 500000
@@ -69,6 +69,11 @@ class _Result(ctypes.Structure):
                 ("best_seed_index", ctypes.c_int32), ("device_ms", ctypes.c_double)]
 
 
+class _DriverParams(ctypes.Structure):
+    _fields_ = [("beam_width", ctypes.c_int32), ("mode", ctypes.c_int32), ("phases", ctypes.c_int32),
+                ("ranks", ctypes.c_int32), ("iterations", ctypes.c_int32), ("time_seed", ctypes.c_uint32)]
+
+
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -89,6 +94,11 @@ _SIGNATURES = {
                                         ctypes.POINTER(_Params), ctypes.POINTER(_Result)]),
     "jssp_solve_batch_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_Instance),
                                            ctypes.POINTER(_Params), ctypes.POINTER(_Result), ctypes.c_int32]),
+    "jssp_solve_seeded": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.POINTER(_Params),
+                                         ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(_Result)]),
+    "jssp_run_passes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.POINTER(_DriverParams),
+                                       ctypes.POINTER(_Result), _I32P]),
+    "jssp_rand_stream": (None, [ctypes.c_uint32, ctypes.c_int32, _I32P]),
     "jssp_engine_set_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p]),
     "jssp_engine_tuned_lanes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "jssp_engine_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
@@ -255,6 +265,43 @@ class Engine:
         return [Result(int(r.makespan), bool(r.found), seqs[i] if r.found else None, int(r.expanded_states),
                        int(r.levels), float(r.device_ms), int(r.best_seed_index)) for i, r in enumerate(rs)]
 
+    def solve_seeded(self, inst: JSSPInstance, beam_width: int, seeds, mode: int = MODE_PMS_GPU,
+                     phases: int = 2, best_known: int = NO_MAKESPAN, flags: int = 0) -> Result:
+        """Multi-pass solve with explicit seeds: seeds[e][i] is the random_seed
+        of beam i in pass e (array of shape (passes, beams)); pass e prunes
+        with the best makespan of the passes before it."""
+        sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.int64) & 0xFFFFFFFF, dtype=np.uint32)
+        if sd.ndim != 2 or sd.size == 0:
+            raise ValueError("seeds must be a non-empty 2-D array (passes, beams)")
+        E, S = sd.shape
+        seq = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
+        r = _Result()
+        r.sequence = _p32(seq)
+        ci = inst._c()
+        pr = _params(beam_width, mode, phases, best_known, 0, S, E, flags)
+        self._check(self.lib.jssp_solve_seeded(self.h, ctypes.byref(ci), ctypes.byref(pr),
+                                               sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(r)))
+        return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                      int(r.levels), float(r.device_ms), int(r.best_seed_index))
+
+    def run_passes(self, inst: JSSPInstance, beam_width: int, ranks: int, iterations: int, time_seed: int,
+                   mode: int = MODE_PMS_GPU, phases: int = 2):
+        """The reference's multi-rank pass loop (gpu/v1/main.cu:55-71,
+        cpu/v1_pms/main.cpp:55-73) with `ranks` ranks, rank r seeded
+        srand(time_seed + r).  Returns (Result, per-pass global best)."""
+        seq = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
+        pass_best = np.zeros(max(1, int(iterations)), dtype=np.int32)
+        r = _Result()
+        r.sequence = _p32(seq)
+        ci = inst._c()
+        d = _DriverParams(int(beam_width), int(mode), int(phases), int(ranks), int(iterations),
+                          int(time_seed) & 0xFFFFFFFF)
+        self._check(self.lib.jssp_run_passes(self.h, ctypes.byref(ci), ctypes.byref(d), ctypes.byref(r),
+                                             _p32(pass_best)))
+        res = Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                     int(r.levels), float(r.device_ms), int(r.best_seed_index))
+        return res, pass_best
+
     def set_autotune(self, on: bool, table_path: Optional[str] = None):
         self._check(self.lib.jssp_engine_set_autotune(self.h, 1 if on else 0,
                                                       table_path.encode() if table_path else None))
@@ -318,6 +365,15 @@ def parse_instance(path: str) -> JSSPInstance:
     if rc != 0:
         raise JsspError(rc, f"malformed instance file {path}")
     return JSSPInstance(mach, dur, name=os.path.basename(path))
+
+
+def rand_stream(seed: int, count: int) -> np.ndarray:
+    """The first `count` values of glibc rand() after srand(seed), computed
+    without touching the process-wide generator."""
+    out = np.zeros(max(0, int(count)), dtype=np.int32)
+    if count > 0:
+        load_library().jssp_rand_stream(int(seed) & 0xFFFFFFFF, int(count), _p32(out))
+    return out
 
 
 def validate_sequence(inst: JSSPInstance, seq: np.ndarray) -> int:

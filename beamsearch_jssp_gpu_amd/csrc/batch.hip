@@ -401,7 +401,8 @@ int solve_batch(jssp_engine* e, int count, const jssp_instance* insts, const jss
 // beam that completes lowers it with atomicMin.  The result is the lowest
 // makespan found (ties: earliest pass, lowest seed index) -- the same as
 // running those solves one by one with the bound carried between passes.
-int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res) {
+int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
+                    const uint32_t* seeds, int32_t* pass_best) {
   const int S = std::max(1, p->n_seeds), E = std::max(1, p->n_passes);
   std::vector<jssp_instance> insts(S, *in);
   std::vector<jssp_params> ps(S, *p);
@@ -416,19 +417,12 @@ int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* 
   int64_t expanded = 0;
   JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
   for (int ep = 0; ep < E; ++ep) {
-    if (ep > 0) {
-      for (int i = 0; i < S; ++i) bp.descs[i].base.lseed = p->seed + (uint32_t)(ep * S + i);
-      JSSP_HIP(e, hipStreamSynchronize(e->stream));
-      std::memcpy(e->hbuf, bp.descs.data(), (size_t)S * sizeof(InstDesc));
-      JSSP_HIP(e, hipMemcpyAsync(bp.d_desc, e->hbuf, (size_t)S * sizeof(InstDesc), hipMemcpyHostToDevice,
-                                 e->stream));
-    } else {
-      for (int i = 0; i < S; ++i) bp.descs[i].base.lseed = p->seed + (uint32_t)i;
-      JSSP_HIP(e, hipStreamSynchronize(e->stream));
-      std::memcpy(e->hbuf, bp.descs.data(), (size_t)S * sizeof(InstDesc));
-      JSSP_HIP(e, hipMemcpyAsync(bp.d_desc, e->hbuf, (size_t)S * sizeof(InstDesc), hipMemcpyHostToDevice,
-                                 e->stream));
-    }
+    for (int i = 0; i < S; ++i)
+      bp.descs[i].base.lseed = seeds ? seeds[(size_t)ep * S + i] : p->seed + (uint32_t)(ep * S + i);
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+    std::memcpy(e->hbuf, bp.descs.data(), (size_t)S * sizeof(InstDesc));
+    JSSP_HIP(e, hipMemcpyAsync(bp.d_desc, e->hbuf, (size_t)S * sizeof(InstDesc), hipMemcpyHostToDevice,
+                               e->stream));
     rc = run_plan(e, bp, true);
     if (rc) return rc;
     rc = fetch_results(e, bp, passes[ep]);
@@ -440,7 +434,7 @@ int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* 
   JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
   int best = p->best_known, best_idx = -1;
   const int* best_seq = nullptr;
-  for (int ep = 0; ep < E; ++ep)
+  for (int ep = 0; ep < E; ++ep) {
     for (int i = 0; i < S; ++i) {
       const HostResults& hr = passes[ep];
       expanded += (int64_t)hr.exp[i];
@@ -450,6 +444,8 @@ int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* 
         best_seq = hr.seq.data() + bp.seq_off[i];
       }
     }
+    if (pass_best) pass_best[ep] = best;
+  }
   res->makespan = best;
   res->found = best_idx >= 0 ? 1 : 0;
   res->best_seed_index = std::max(0, best_idx);
@@ -476,6 +472,21 @@ int jssp_solve_batch_ex(jssp_engine* e, int32_t count, const jssp_instance* inst
   if (!e || count < 0 || (count > 0 && (!insts || !params || !results))) return JSSP_ERR_INVALID;
   std::lock_guard<std::recursive_mutex> lk(e->mu);
   return jssp::solve_batch(e, count, insts, params, results, use_graph != 0);
+}
+
+int jssp_solve_seeded(jssp_engine* e, const jssp_instance* in, const jssp_params* p, const uint32_t* seeds,
+                      jssp_result* res) {
+  if (!e || !in || !p || !seeds || !res) return JSSP_ERR_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(e->mu);
+  int rc = jssp::check_instance(e, in);
+  if (rc) return rc;
+  rc = jssp::check_params(e, in, p);
+  if (rc) return rc;
+  if (p->mode != JSSP_MODE_ICH_DET && p->mode != JSSP_MODE_PMS_GPU)
+    return jssp::set_error(e, JSSP_ERR_INVALID, "jssp_solve_seeded", "seed lists apply to modes ICH_DET and PMS_GPU");
+  JSSP_HIP(e, hipSetDevice(e->device));
+  res->best_seed_index = 0;
+  return jssp::solve_multiseed(e, in, p, res, seeds, nullptr);
 }
 
 int jssp_engine_set_autotune(jssp_engine* e, int32_t on, const char* table_path) {

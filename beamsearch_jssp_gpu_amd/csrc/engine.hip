@@ -421,8 +421,8 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
 
 // ICH_EXACT / PMS_EXACT: the device expands every child of every parent
 // (cmax, job, machine per erase-order ready position); the host consumes the
-// process-global rand() stream in the reference's order and replays the
-// libstdc++ bounded heap (cpu/common/jssp_types.h:47-58) on order-isomorphic
+// rand() stream (process-wide, or private and seeded) in the reference's
+// order and replays the libstdc++ bounded heap (cpu/common/jssp_types.h:47-58) on order-isomorphic
 // keys; the device then materialises the heap's array in order, which is the
 // parent order of the next level (cpu/v0_ich/solver.cpp:22,42).
 struct HeapEntry {
@@ -455,8 +455,10 @@ void set_exact_level(LevelArgs& a, const Work& w, int level) {
   a.beam_next = w.beam[(level + 1) & 1];
 }
 
-int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
-                int32_t* trail_out, int32_t* pcount_out) {
+}  // namespace
+
+int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, RandStream& rs,
+                jssp_result* res, int32_t* trail_out, int32_t* pcount_out) {
   const bool ich = (p->mode == JSSP_MODE_ICH_EXACT);
   const Shape s = make_shape(in, p, 64);
   Work w;
@@ -475,7 +477,6 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, j
   int* h_r = reinterpret_cast<int*>(((uintptr_t)(h_mc + N) + 15) & ~(uintptr_t)15);
   int* h_order = h_r + s.K;
 
-  if (!(p->flags & JSSP_FLAG_GLOBAL_RAND)) srand(p->seed);
   LevelArgs a = base_args(s, w, p);
   JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
   JSSP_HIP(e, launch_init(a, e->stream));
@@ -502,7 +503,7 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, j
       if (ich) {
         for (int qq = 0; qq < r; ++qq) {
           HeapEntry h;
-          h.ikey = 100ll * h_cmax[base + qq] + (rand() % 100);  // cpu/v0_ich/solver.cpp:35
+          h.ikey = 100ll * h_cmax[base + qq] + (rs.next() % 100);  // cpu/v0_ich/solver.cpp:35
           h.dkey = 0.0;
           h.x = (int)(base + qq);
           ++expanded;
@@ -510,12 +511,12 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, j
         }
       } else {
         if (r <= 0) continue;
-        const int pivot = rand() % r;                          // cpu/v1_pms/solver.cpp:32
+        const int pivot = rs.next() % r;                       // cpu/v1_pms/solver.cpp:32
         const int tmc = h_mc[base + pivot];
         for (int qq = 0; qq < r; ++qq) {
           if (h_mc[base + qq] != tmc) continue;
           const int cm = h_cmax[base + qq];
-          const double noise = (double)(rand() % 1000) / 1000000.0;
+          const double noise = (double)(rs.next() % 1000) / 1000000.0;
           HeapEntry h;
           h.ikey = 0;
           h.dkey = ((double)cm / (double)ns) + noise;
@@ -566,9 +567,6 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, j
   return JSSP_OK;
 }
 
-}  // namespace
-
-int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res);
 
 int solve_any(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
               int32_t* trail_out, int32_t* pcount_out) {
@@ -579,9 +577,12 @@ int solve_any(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jss
   if (!res) return set_error(e, JSSP_ERR_INVALID, "jssp_solve", "null result");
   JSSP_HIP(e, hipSetDevice(e->device));
   res->best_seed_index = 0;
-  if (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT)
-    return solve_exact(e, in, p, res, trail_out, pcount_out);
-  if (std::max(1, p->n_seeds) > 1 || std::max(1, p->n_passes) > 1) return solve_multiseed(e, in, p, res);
+  if (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT) {
+    RandStream rs((p->flags & JSSP_FLAG_GLOBAL_RAND) != 0, p->seed);
+    return solve_exact(e, in, p, rs, res, trail_out, pcount_out);
+  }
+  if (std::max(1, p->n_seeds) > 1 || std::max(1, p->n_passes) > 1)
+    return solve_multiseed(e, in, p, res, nullptr, nullptr);
   return solve_device(e, in, p, p->seed, p->best_known, res, trail_out, pcount_out);
 }
 

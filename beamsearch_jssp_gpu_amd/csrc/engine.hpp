@@ -10,6 +10,7 @@
 
 #include "../../include/jssp.h"
 #include "level_kernels.hpp"
+#include "rand_stream.hpp"
 
 // Kernel kinds timed by the optional profiler.
 enum { kKindScore = 0, kKindFilter = 1, kKindRank = 2, kNumKinds = 3 };
@@ -67,6 +68,16 @@ int ensure_host_staging(jssp_engine* e, size_t bytes);
 int check_instance(jssp_engine* e, const jssp_instance* in);
 int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p);
 void release_batch_state(jssp_engine* e);
+
+// ICH_EXACT / PMS_EXACT solve drawing the reference's rand() values from rs.
+int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, RandStream& rs,
+                jssp_result* res, int32_t* trail_out, int32_t* pcount_out);
+// n_seeds beams per pass, n_passes passes; pass e prunes with the best
+// makespan found before it.  Beam i of pass e uses seeds[e * n_seeds + i]
+// (seed + e * n_seeds + i when seeds is null); pass_best[e], when given,
+// receives the best makespan after pass e.
+int solve_multiseed(jssp_engine* e, const jssp_instance* in, const jssp_params* p, jssp_result* res,
+                    const uint32_t* seeds, int32_t* pass_best);
 
 }  // namespace jssp
 

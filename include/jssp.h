@@ -125,6 +125,48 @@ int jssp_solve(jssp_engine* engine, const jssp_instance* inst, const jssp_params
 int jssp_solve_trace(jssp_engine* engine, const jssp_instance* inst, const jssp_params* params,
                      jssp_result* result, int32_t* trail, int32_t* pcount);
 
+/* Multi-seed / multi-pass solve with explicit level seeds: beam i of pass e
+ * uses seeds[e * n_seeds + i] (n_seeds * n_passes entries) instead of
+ * seed + e * n_seeds + i.  This is the reference drivers' loop
+ * (gpu/v1/main.cu:60-71, gpu/v2_twophases/main.cu:60-71: seed = rand() per
+ * iteration on each rank, best_known = the all-reduced minimum of the
+ * previous iteration) with the ranks as the beams of a pass.  Modes ICH_DET
+ * and PMS_GPU. */
+int jssp_solve_seeded(jssp_engine* engine, const jssp_instance* inst, const jssp_params* params,
+                      const uint32_t* seeds, jssp_result* result);
+
+/* The reference's multi-rank drivers on one device: gpu/v1/main.cu:55-71,
+ * gpu/v2_twophases/main.cu:55-71 (PMS_GPU) and cpu/v1_pms/main.cpp:55-73
+ * (PMS_EXACT), also the per-instance loops of gpu/v1/benchmark.cu:52-66 and
+ * cpu/v1_pms/benchmark.cpp:48-66.  Rank r draws from its own rand() stream,
+ * seeded srand(time_seed + r); each of `iterations` passes runs one solve per
+ * rank with best_known = the minimum over every rank of the passes before it
+ * (MPI_Allreduce MIN), starting from JSSP_NO_MAKESPAN.
+ *   PMS_GPU:   the rank's solve seed is its next rand() value (seed = rand();
+ *              run_gpu_beam_search_v1/_v2(inst, K, global_best, seed)); the
+ *              ranks of a pass run as the beams of one batched launch.
+ *   PMS_EXACT: beam_search_pms(inst, K, global_best) consuming the rank's
+ *              stream (cpu/v1_pms/solver.cpp:32,52).
+ * result: makespan = the final global best, sequence = the first (pass, rank)
+ * solve that reached it, best_seed_index = pass * ranks + rank,
+ * expanded_states summed over all solves.  pass_best[e] (`iterations`
+ * entries, may be NULL) = the global best after pass e.  The process-wide
+ * rand() state is not touched. */
+typedef struct jssp_driver_params {
+  int32_t beam_width;  /* K */
+  int32_t mode;        /* JSSP_MODE_PMS_GPU or JSSP_MODE_PMS_EXACT */
+  int32_t phases;      /* PMS_GPU: 1 = gpu/v1 driver, 2 = gpu/v2_twophases driver */
+  int32_t ranks;       /* MPI world size (>= 1) */
+  int32_t iterations;  /* passes (>= 1) */
+  uint32_t time_seed;  /* the time(NULL) value the reference driver read */
+} jssp_driver_params;
+int jssp_run_passes(jssp_engine* engine, const jssp_instance* inst, const jssp_driver_params* params,
+                    jssp_result* result, int32_t* pass_best);
+
+/* out[i] = the i-th value of rand() after srand(seed) (glibc's generator),
+ * computed without touching the process-wide rand() state. */
+void jssp_rand_stream(uint32_t seed, int32_t count, int32_t* out);
+
 /* Batched solve: advances all `count` instances together, one level per
  * step; a step is three launches (score, filter, rank) whose grids cover every
  * instance, replayed from a captured hipGraph.  Results equal those of

@@ -30,6 +30,19 @@
 //   orc_gpu_score   gpu/v1/solver.cu:121-127; pinned by exact FP32 values
 //                   computed from the definition in numpy.
 //   orc_bruteforce  exhaustive search over all operation sequences.
+//   orc_glibc_rand  glibc's TYPE_3 random(), the generator behind rand()
+//                   (stdlib/random_r.c, glibc 2.35).  Pinned against the C
+//                   library's own srand()/rand() in tests.
+//   orc_pms_driver  cpu/v1_pms/main.cpp:55-73 over orc_glibc_rand streams.
+//                   Pinned: one rank and one pass reduce to orc_pms_exact after
+//                   srand(time_seed) (itself pinned by the goldens); the
+//                   reference's own beam_search_pms run under the same
+//                   per-rank streams (oracle/_ref) gives the same values.
+//   orc_gpu_driver  gpu/v1/main.cu:55-71 over orc_glibc_rand streams and
+//                   orc_gpu_solve.  Pinned: one rank and one pass reduce to
+//                   orc_gpu_solve with seed = the first rand() after
+//                   srand(time_seed); later passes prune with the earlier
+//                   minimum, checked against independent orc_gpu_solve calls.
 //
 // Float rules: compile without -ffast-math and with -ffp-contract=off; the
 // FP32 score is evaluated as (float)cmax / (float)ns + (float)(h % 1000) / 1e9f
@@ -156,11 +169,43 @@ int orc_ich_exact(int n, int m, const int* mach, const int* dur, int beam_width,
   return best;
 }
 
-// BS-PMS, cpu/v1_pms/solver.cpp:8-70 (one pass).  Consumes the global rand().
-int orc_pms_exact(int n, int m, const int* mach, const int* dur, int beam_width,
-                  int best_known, unsigned srand_seed, int use_srand,
-                  long long* expanded) {
-  if (use_srand) srand(srand_seed);
+}  // extern "C"
+
+namespace {
+
+// glibc random() over the TYPE_3 state that rand()/srand() use (glibc
+// stdlib/random_r.c; glibc 2.35 in this image): 31 words, state[0] = seed
+// (0 -> 1), state[i] = 16807 * state[i-1] mod (2^31 - 1) by Schrage's method,
+// then the additive feedback state[f] += state[r] with f three ahead of r,
+// output state[f] >> 1, and the first 310 outputs discarded.
+struct GlibcRand {
+  uint32_t st[31];
+  int f = 3, r = 0;
+  explicit GlibcRand(unsigned seed) {
+    if (seed == 0) seed = 1;
+    st[0] = seed;
+    int32_t word = (int32_t)seed;
+    for (int i = 1; i < 31; ++i) {
+      long hi = word / 127773, lo = word % 127773;
+      word = (int32_t)(16807 * lo - 2836 * hi);
+      if (word < 0) word += 2147483647;
+      st[i] = (uint32_t)word;
+    }
+    for (int i = 0; i < 310; ++i) next();
+  }
+  int next() {
+    st[f] += st[r];
+    const int out = (int)(st[f] >> 1);
+    f = (f + 1) % 31;
+    r = (r + 1) % 31;
+    return out;
+  }
+};
+
+// BS-PMS, cpu/v1_pms/solver.cpp:8-70 (one pass); rnd() stands for rand().
+template <class Rand>
+int pms_pass(int n, int m, const int* mach, const int* dur, int beam_width, int best_known, Rand&& rnd,
+             long long* expanded) {
   RefInst in = make_ref_inst(n, m, mach, dur);
   const int total = n * m;
   std::vector<RefState> cand, next;
@@ -173,7 +218,7 @@ int orc_pms_exact(int n, int m, const int* mach, const int* dur, int beam_width,
       const std::vector<int>& ready = cand[ic].c;
       int nb = (int)ready.size();
       if (nb <= 0) continue;
-      int pivot = rand() % nb;
+      int pivot = rnd() % nb;
       int tmc = in.mc[ready[pivot]];
       for (int r = 0; r < nb; ++r) {
         if (in.mc[ready[r]] != tmc) continue;
@@ -182,7 +227,7 @@ int orc_pms_exact(int n, int m, const int* mach, const int* dur, int beam_width,
         auto it = std::find(child.c.begin(), child.c.end(), op);
         if (it != child.c.end()) child.c.erase(it);
         ref_schedule(op, in, child);
-        double noise = (double)(rand() % 1000) / 1000000.0;
+        double noise = (double)(rnd() % 1000) / 1000000.0;
         child.score = ((double)child.cmax / child.seq.size()) + noise;
         ++ex;
         if (child.cmax < best_known) ref_insert(next, child, beam_width);
@@ -193,6 +238,61 @@ int orc_pms_exact(int n, int m, const int* mach, const int* dur, int beam_width,
   }
   if (expanded) *expanded = ex;
   return best;
+}
+
+}  // namespace
+
+extern "C" {
+
+// BS-PMS, cpu/v1_pms/solver.cpp:8-70 (one pass).  Consumes the global rand().
+int orc_pms_exact(int n, int m, const int* mach, const int* dur, int beam_width,
+                  int best_known, unsigned srand_seed, int use_srand,
+                  long long* expanded) {
+  if (use_srand) srand(srand_seed);
+  return pms_pass(n, m, mach, dur, beam_width, best_known, [] { return rand(); }, expanded);
+}
+
+// out[i] = the i-th rand() after srand(seed), from the restated generator.
+void orc_glibc_rand(unsigned seed, int count, int* out) {
+  GlibcRand g(seed);
+  for (int i = 0; i < count; ++i) out[i] = g.next();
+}
+
+// The PMS driver loop, cpu/v1_pms/main.cpp:55-73: rank r seeds its stream
+// with srand(time_seed + r); each iteration every rank runs
+// beam_search_pms(inst, K, global_best), local_best = min(local_best,
+// result), then global_best = MPI_Allreduce(local_best, MIN).
+// pass_best[e] = global_best after iteration e; *best_index = e * ranks + r
+// of the first solve reaching the final value (-1 if none);
+// *expanded = children summed over all solves.  Returns global_best.
+int orc_pms_driver(int n, int m, const int* mach, const int* dur, int beam_width, int ranks,
+                   int iterations, unsigned time_seed, int* pass_best, int* best_index,
+                   long long* expanded) {
+  std::vector<GlibcRand> streams;
+  for (int r = 0; r < ranks; ++r) streams.emplace_back(time_seed + (unsigned)r);
+  std::vector<int> local(ranks, kNoMakespan);
+  int global = kNoMakespan, bi = -1;
+  long long ex = 0;
+  for (int it = 0; it < iterations; ++it) {
+    for (int r = 0; r < ranks; ++r) {
+      long long e1 = 0;
+      GlibcRand& g = streams[r];
+      const int ms = pms_pass(n, m, mach, dur, beam_width, global, [&g] { return g.next(); }, &e1);
+      ex += e1;
+      if (ms < local[r]) local[r] = ms;
+    }
+    int red = kNoMakespan;
+    for (int r = 0; r < ranks; ++r) red = std::min(red, local[r]);
+    if (red < global) {
+      for (int r = 0; r < ranks && bi < it * ranks; ++r)
+        if (local[r] == red) bi = it * ranks + r;
+    }
+    global = red;
+    if (pass_best) pass_best[it] = global;
+  }
+  if (best_index) *best_index = bi;
+  if (expanded) *expanded = ex;
+  return global;
 }
 
 }  // extern "C"
@@ -405,6 +505,47 @@ static int bf_rec(int n, int m, const int* mach, const int* dur, std::vector<int
     mr[mc] = smr; jr[j] = sjr; nx[j]--;
   }
   return best;
+}
+
+// The GPU driver loop, gpu/v1/main.cu:55-71 and gpu/v2_twophases/main.cu:
+// 55-71: rank r seeds its stream with srand(time_seed + r); each iteration
+// every rank draws seed = rand() and runs the PMS_GPU beam search with
+// best_known = global_best; local_best = min(local_best, result), then
+// global_best = MPI_Allreduce(local_best, MIN).  Outputs as orc_pms_driver;
+// seq_out (n*m) receives the sequence of the first solve reaching the final
+// value.
+int orc_gpu_driver(int n, int m, const int* mach, const int* dur, int beam_width, int ranks,
+                   int iterations, unsigned time_seed, int* pass_best, int* best_index,
+                   long long* expanded, int* seq_out) {
+  std::vector<GlibcRand> streams;
+  for (int r = 0; r < ranks; ++r) streams.emplace_back(time_seed + (unsigned)r);
+  std::vector<int> local(ranks, kNoMakespan), seq((size_t)n * m);
+  int global = kNoMakespan, bi = -1;
+  long long ex = 0;
+  for (int it = 0; it < iterations; ++it) {
+    int running = global;
+    for (int r = 0; r < ranks; ++r) {
+      const unsigned seed = (unsigned)streams[r].next();
+      long long e1 = 0;
+      int found = 0;
+      const int ms = orc_gpu_solve(n, m, mach, dur, beam_width, 2, global, seed, seq.data(), &e1, &found,
+                                   nullptr, nullptr);
+      ex += e1;
+      if (ms < local[r]) local[r] = ms;
+      if (found && ms < running) {
+        running = ms;
+        bi = it * ranks + r;
+        if (seq_out) std::memcpy(seq_out, seq.data(), seq.size() * sizeof(int));
+      }
+    }
+    int red = kNoMakespan;
+    for (int r = 0; r < ranks; ++r) red = std::min(red, local[r]);
+    global = red;
+    if (pass_best) pass_best[it] = global;
+  }
+  if (best_index) *best_index = bi;
+  if (expanded) *expanded = ex;
+  return global;
 }
 
 // Exhaustive minimum makespan over every operation sequence (tiny instances).

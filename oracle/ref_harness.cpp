@@ -60,6 +60,38 @@ int ref_pms(int n, int m, const int* mach, const int* dur, int beam_width, int b
   return beam_search_pms(inst, beam_width, best_known);
 }
 
+// The loop of cpu/v1_pms/main.cpp:55-73 with `ranks` MPI ranks replayed in
+// one process: each rank's rand() stream is a glibc state buffer seeded
+// initstate(time_seed + r) and made current with setstate() around that
+// rank's beam_search_pms call, so every rank sees exactly the values its own
+// process would (srand(time + rank) then rand()...).  pass_best[it] = the
+// all-reduced minimum after iteration it.  The process-wide state is
+// restored afterwards.
+int ref_pms_driver(int n, int m, const int* mach, const int* dur, int beam_width, int ranks,
+                   int iterations, unsigned time_seed, int* pass_best) {
+  JSSPInstance inst;
+  if (!load(n, m, mach, dur, inst) || ranks < 1 || ranks > 64) return -1;
+  static char states[64][128];
+  char* saved = initstate(time_seed, states[0], 128);
+  for (int r = 1; r < ranks; ++r) initstate(time_seed + (unsigned)r, states[r], 128);
+  int local[64];
+  for (int r = 0; r < ranks; ++r) local[r] = 500000;
+  int global_best = 500000;
+  for (int it = 0; it < iterations; ++it) {
+    for (int r = 0; r < ranks; ++r) {
+      setstate(states[r]);
+      int makespan = beam_search_pms(inst, beam_width, global_best);
+      if (makespan < local[r]) local[r] = makespan;
+    }
+    int red = 500000;
+    for (int r = 0; r < ranks; ++r) red = red < local[r] ? red : local[r];
+    global_best = red;
+    if (pass_best) pass_best[it] = global_best;
+  }
+  setstate(saved);
+  return global_best;
+}
+
 // Parses a file with the reference parser and returns it 0-based
 // (machine = num_mc - 1), job-major.  Returns n*m, or -1 on failure / when
 // cap is too small.

@@ -1,0 +1,126 @@
+"""The reference's multi-rank pass drivers on one device (jssp_run_passes,
+jssp_solve_seeded): each result must equal the oracle's restatement of the
+driver loop (gpu/v1/main.cu:55-71, cpu/v1_pms/main.cpp:55-73) run on the
+same rank streams."""
+import numpy as np
+import pytest
+
+import beamsearch_jssp_gpu_amd as bs
+from testlib import golden_instances, orc_gpu, orc_gpu_driver, orc_pms_driver, orc_rand, splitmix_instance
+
+pytestmark = pytest.mark.gpu
+
+
+def _b(i):
+    return bs.JSSPInstance(i.machine, i.duration, i.name)
+
+
+@pytest.mark.parametrize("phases", [1, 2])
+def test_gpu_driver_equals_oracle_loop(engine, phases):
+    inst = splitmix_instance(31, 15, 10, 1, 99)
+    K, R, E, t = 64, 4, 3, 1700000000
+    print("gpu driver", phases, flush=True)
+    got, pass_best = engine.run_passes(_b(inst), K, R, E, t, mode=bs.MODE_PMS_GPU, phases=phases)
+    want = orc_gpu_driver(inst, K, R, E, t)
+    assert got.makespan == want["makespan"]
+    np.testing.assert_array_equal(pass_best, want["pass_best"])
+    assert got.expanded_states == want["expanded"]
+    assert got.best_seed_index == max(0, want["best_index"])
+    assert got.found == (want["best_index"] >= 0)
+    if got.found:
+        np.testing.assert_array_equal(got.sequence, want["sequence"])
+        assert bs.validate_sequence(_b(inst), got.sequence) == got.makespan
+
+
+def test_pms_exact_driver_equals_oracle_loop(engine):
+    inst = golden_instances()["la01"]
+    K, R, E, t = 16, 3, 3, 4242
+    print("pms driver", flush=True)
+    got, pass_best = engine.run_passes(_b(inst), K, R, E, t, mode=bs.MODE_PMS_EXACT)
+    want = orc_pms_driver(inst, K, R, E, t)
+    assert got.makespan == want["makespan"]
+    np.testing.assert_array_equal(pass_best, want["pass_best"])
+    assert got.expanded_states == want["expanded"]
+    assert got.best_seed_index == max(0, want["best_index"])
+    assert bs.validate_sequence(_b(inst), got.sequence) == got.makespan
+
+
+def test_seeded_passes_equal_single_solves(engine):
+    """jssp_solve_seeded: beam i of pass e uses seeds[e][i] and prunes with
+    the best of the passes before e -- the same as single solves."""
+    inst = splitmix_instance(12, 12, 8, 1, 99)
+    seeds = np.array([[5, 2**31 + 7, 999], [123456789, 0, 2**32 - 1]], dtype=np.int64)
+    K = 32
+    for mode in (bs.MODE_PMS_GPU, bs.MODE_ICH_DET):
+        got = engine.solve_seeded(_b(inst), K, seeds, mode=mode)
+        bound, best_idx, best_seq, exp = bs.NO_MAKESPAN, -1, None, 0
+        for e in range(seeds.shape[0]):
+            pb = bound
+            for i in range(seeds.shape[1]):
+                r = engine.solve(_b(inst), K, mode=mode, best_known=pb, seed=int(seeds[e, i]))
+                exp += r.expanded_states
+                if r.found and r.makespan < bound:
+                    bound, best_idx, best_seq = r.makespan, e * seeds.shape[1] + i, r.sequence
+        assert got.makespan == bound and got.expanded_states == exp
+        assert got.best_seed_index == max(0, best_idx)
+        np.testing.assert_array_equal(got.sequence, best_seq)
+    # and one of them against the oracle directly
+    w = orc_gpu(inst, K, bs.MODE_PMS_GPU, bs.NO_MAKESPAN, int(seeds[0, 1]), trace=False)
+    r = engine.solve(_b(inst), K, mode=bs.MODE_PMS_GPU, seed=int(seeds[0, 1]))
+    assert r.makespan == w["makespan"] and r.expanded_states == w["expanded"]
+
+
+def test_driver_seeds_are_rank_streams(engine):
+    """One rank, one pass: the solve seed is the first rand() after
+    srand(time_seed), as gpu/v1/main.cu:60-63."""
+    inst = splitmix_instance(3, 10, 6, 1, 99)
+    t = 31337
+    got, _ = engine.run_passes(_b(inst), 32, 1, 1, t, mode=bs.MODE_PMS_GPU, phases=2)
+    want = engine.solve(_b(inst), 32, mode=bs.MODE_PMS_GPU, seed=int(orc_rand(t, 1)[0]))
+    assert got.makespan == want.makespan and got.expanded_states == want.expanded_states
+    np.testing.assert_array_equal(got.sequence, want.sequence)
+
+
+def test_driver_argument_errors(engine):
+    inst = splitmix_instance(3, 6, 4, 1, 99)
+    for kw in (dict(mode=bs.MODE_ICH_DET), dict(mode=bs.MODE_PMS_GPU, ranks=0), dict(mode=bs.MODE_PMS_GPU,
+                                                                                       iterations=0)):
+        args = dict(ranks=2, iterations=1, mode=bs.MODE_PMS_GPU)
+        args.update(kw)
+        with pytest.raises(bs.JsspError):
+            engine.run_passes(_b(inst), 8, args["ranks"], args["iterations"], 1, mode=args["mode"])
+
+
+def test_lawrence_runner_rows_equal_oracle(engine, tmp_path, capsys):
+    """bench_gpu_v2 / bench_pms on la01-la02 (instance files written from the
+    fixture): every BestSpan equals the oracle's driver loop with the
+    runner's K rule and seeds."""
+    import os
+    from beamsearch_jssp_gpu_amd import drivers
+    from testlib import load_json
+    law = load_json("lawrence.json")
+    for name in ("la01", "la02"):
+        d = law[name]
+        with open(tmp_path / f"{name}.txt", "w") as f:
+            f.write(f"{d['n']} {d['m']}\n")
+            for j in range(d["n"]):
+                row = range(j * d["m"], (j + 1) * d["m"])
+                f.write(" ".join(f"{d['machine'][i]} {d['duration'][i]}" for i in row) + "\n")
+    insts = golden_instances()
+    cwd = os.getcwd()
+    for cmd, iters in (("bench_gpu_v2", 3), ("bench_pms", 2)):
+        print(cmd, flush=True)
+        try:
+            assert drivers.main([cmd, "--dir", str(tmp_path), "--only", "01,02", "--iterations", str(iters),
+                                 "--time-seed", "20240601", "--ranks", "2"]) == 0
+        finally:
+            os.chdir(cwd)
+        rows = [l.split() for l in capsys.readouterr().out.splitlines() if l.startswith("1000 ")]
+        assert [r[1] for r in rows] == ["./la01.txt", "./la02.txt"]
+        for r, name in zip(rows, ("la01", "la02")):
+            assert r[2:5] == ["10", "5", "52"]
+            if cmd == "bench_pms":
+                want = orc_pms_driver(insts[name], 1000, 2, iters, 20240601)["makespan"]
+            else:
+                want = orc_gpu_driver(insts[name], 1000, 2, iters, 20240601)["makespan"]
+            assert int(r[5]) == want, (cmd, name)

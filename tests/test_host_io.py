@@ -97,3 +97,36 @@ def test_validator_and_lower_bound():
     bad = np.arange(24, 0, -1, dtype=np.int32)
     assert bs.validate_sequence(inst, bad) == -1
     assert bs.lower_bound(inst) == lower_bound(inst)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 12345, 1700000003, 2**31 + 99, 2**32 - 1])
+def test_rank_rand_streams_match_libc(seed):
+    """jssp_rand_stream (the private per-rank streams of jssp_run_passes)
+    gives srand(seed); rand()... of the C library and of the restated
+    generator, and leaves the process-wide stream where it was."""
+    from testlib import libc_rand, orc_rand
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.rand.restype = ctypes.c_int
+    libc.srand(777)
+    got = bs.rand_stream(seed, 500)
+    after = libc.rand()
+    np.testing.assert_array_equal(got, orc_rand(seed, 500))
+    np.testing.assert_array_equal(got, libc_rand(seed, 500))
+    assert after == int(libc_rand(777, 1)[0])
+
+
+def test_lawrence_runner_skips_missing_files(tmp_path, capsys):
+    """With no instance files in reach the runner prints the reference's
+    header and one SKIP line per id, and touches no device."""
+    from beamsearch_jssp_gpu_amd import drivers
+    cwd = os.getcwd()
+    try:
+        assert drivers.main(["bench_gpu_v1", "--dir", str(tmp_path), "--time-seed", "1"]) == 0
+    finally:
+        os.chdir(cwd)
+    out = capsys.readouterr().out.splitlines()
+    assert out[1].split() == ["Kbest", "Instance", "Jobs", "Machines", "Ops", "BestSpan", "Time(s)"]
+    skips = [l for l in out if l.startswith("SKIP")]
+    assert skips[0] == "SKIP la01  (file not found)" and len(skips) == 40
+    assert [drivers.lawrence_beam_width(i) for i in (0, 9, 10, 39)] == [1000, 1000, 5000, 5000]

@@ -11,14 +11,18 @@
   constant or shift breaks the round trip);
 * orc_gpu_solve's operation-level child generation against orc_ich_exact
   (same ready order and children; only the score differs), via K=1 greedy
-  equality on instances where the rand noise cannot matter.
+  equality on instances where the rand noise cannot matter;
+* the restated glibc generator against the C library's own srand()/rand();
+* the multi-rank driver loops: one rank and one pass reduce to the pinned
+  single solves, and the PMS loop equals the reference's own solver run
+  under per-rank glibc states (oracle/_ref).
 """
 import numpy as np
 import pytest
 
-from testlib import (Inst, golden_instances, load_json, lower_bound, oracle, orc_bruteforce, orc_gpu,
-                     orc_ich, orc_pms, orc_simulate, random_instance, ref_lib, splitmix_instance,
-                     valid_trail_entries)
+from testlib import (Inst, golden_instances, libc_rand, load_json, lower_bound, oracle, orc_bruteforce,
+                     orc_gpu, orc_gpu_driver, orc_ich, orc_pms, orc_pms_driver, orc_rand, orc_simulate, p32,
+                     random_instance, ref_lib, splitmix_instance, valid_trail_entries)
 
 ICH_DET, PMS_GPU = 1, 2
 NO = 500000
@@ -209,3 +213,53 @@ def test_oracle_score_is_fp32_division_plus_noise():
         want = np.float32(np.float32(cmax) / np.float32(ns)) + np.float32(np.float32(h % 1000) / np.float32(1e9))
         got = np.float32(lib.orc_gpu_score(cmax, ns, h))
         assert got.view(np.uint32) == np.float32(want).view(np.uint32), (cmax, ns, h)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 12345, 1700000000, 2**31 - 1, 2**31, 2**31 + 12345, 2**32 - 1])
+def test_oracle_rand_is_glibc_rand(seed):
+    """The restated TYPE_3 generator gives the C library's own rand() values
+    (seed 0 maps to 1; seeds >= 2^31 go through the signed seeding path)."""
+    np.testing.assert_array_equal(orc_rand(seed, 1000), libc_rand(seed, 1000))
+
+
+def test_pms_driver_reduces_to_single_pass():
+    inst = golden_instances()["la01"]
+    for t in (5, 1234567):
+        d = orc_pms_driver(inst, 16, 1, 1, t)
+        ms, ex = orc_pms(inst, 16, NO, t)
+        assert d["makespan"] == ms and d["expanded"] == ex and list(d["pass_best"]) == [ms]
+        assert d["best_index"] == (0 if ms < NO else -1)
+
+
+@pytest.mark.skipif(ref_lib() is None, reason="oracle/_ref not built (no reference tree)")
+def test_pms_driver_matches_reference_solver_with_rank_streams():
+    ref = ref_lib()
+    insts = golden_instances()
+    for name, K, R, E, t in [("la01", 8, 3, 4, 1700000000), ("la06", 16, 2, 3, 42), ("r10x5s1", 4, 4, 3, 7)]:
+        inst = insts[name]
+        want = np.zeros(E, np.int32)
+        got = orc_pms_driver(inst, K, R, E, t)
+        rv = ref.ref_pms_driver(inst.n, inst.m, p32(inst.machine), p32(inst.duration), K, R, E, t, p32(want))
+        assert got["makespan"] == rv, name
+        np.testing.assert_array_equal(got["pass_best"], want)
+        assert np.all(np.diff(want) <= 0)
+
+
+def test_gpu_driver_reduces_to_single_solves():
+    inst = splitmix_instance(21, 8, 5, 1, 99)
+    t = 99
+    d = orc_gpu_driver(inst, 16, 1, 1, t)
+    w = orc_gpu(inst, 16, PMS_GPU, NO, int(orc_rand(t, 1)[0]), trace=False)
+    assert d["makespan"] == w["makespan"] and d["expanded"] == w["expanded"]
+    np.testing.assert_array_equal(d["sequence"], w["sequence"])
+    # two ranks, three passes: pass e prunes with the minimum of passes < e
+    R, E = 2, 3
+    d = orc_gpu_driver(inst, 8, R, E, t)
+    seeds = np.stack([orc_rand(t + r, E) for r in range(R)], axis=1)  # [pass, rank]
+    bound, exp, mins = NO, 0, []
+    for e in range(E):
+        res = [orc_gpu(inst, 8, PMS_GPU, bound, int(seeds[e, r]), trace=False) for r in range(R)]
+        exp += sum(x["expanded"] for x in res)
+        bound = min([bound] + [x["makespan"] for x in res])
+        mins.append(bound)
+    assert d["expanded"] == exp and list(d["pass_best"]) == mins and d["makespan"] == bound

@@ -47,6 +47,12 @@ def oracle():
         lib.orc_noise_hash.restype = ctypes.c_uint32
         lib.orc_gpu_score.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32]
         lib.orc_gpu_score.restype = ctypes.c_float
+        lib.orc_glibc_rand.argtypes = [ctypes.c_uint, ctypes.c_int, I32P]
+        lib.orc_glibc_rand.restype = None
+        lib.orc_pms_driver.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint, I32P, ip, ll]
+        lib.orc_gpu_driver.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_uint, I32P, ip, ll, I32P]
         _oracle = lib
     return _oracle
 
@@ -58,6 +64,8 @@ def ref_lib():
     lib = ctypes.CDLL(REF_SO)
     lib.ref_ich.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_uint]
     lib.ref_pms.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int, ctypes.c_uint]
+    lib.ref_pms_driver.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_uint, I32P]
     return lib
 
 
@@ -158,6 +166,39 @@ def orc_gpu(inst: Inst, K: int, mode: int, best_known: int, seed: int, trace=Tru
                                 p32(trail) if trace else None, p32(pcount) if trace else None)
     return {"makespan": ms, "found": bool(found.value), "sequence": seq if found.value else None,
             "expanded": ex.value, "trail": trail, "pcount": pcount}
+
+
+def orc_rand(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count, np.int32)
+    oracle().orc_glibc_rand(seed & 0xFFFFFFFF, count, p32(out))
+    return out
+
+
+def libc_rand(seed: int, count: int) -> np.ndarray:
+    """srand(seed) then `count` rand() calls of the C library itself."""
+    libc = ctypes.CDLL(None)
+    libc.srand.argtypes = [ctypes.c_uint]
+    libc.rand.restype = ctypes.c_int
+    libc.srand(seed & 0xFFFFFFFF)
+    return np.array([libc.rand() for _ in range(count)], np.int32)
+
+
+def orc_pms_driver(inst: Inst, K: int, ranks: int, iterations: int, time_seed: int):
+    pb = np.zeros(iterations, np.int32)
+    bi, ex = ctypes.c_int(), ctypes.c_longlong()
+    ms = oracle().orc_pms_driver(inst.n, inst.m, p32(inst.machine), p32(inst.duration), K, ranks, iterations,
+                                 time_seed & 0xFFFFFFFF, p32(pb), ctypes.byref(bi), ctypes.byref(ex))
+    return {"makespan": ms, "pass_best": pb, "best_index": bi.value, "expanded": ex.value}
+
+
+def orc_gpu_driver(inst: Inst, K: int, ranks: int, iterations: int, time_seed: int):
+    pb = np.zeros(iterations, np.int32)
+    seq = np.zeros(inst.n * inst.m, np.int32)
+    bi, ex = ctypes.c_int(), ctypes.c_longlong()
+    ms = oracle().orc_gpu_driver(inst.n, inst.m, p32(inst.machine), p32(inst.duration), K, ranks, iterations,
+                                 time_seed & 0xFFFFFFFF, p32(pb), ctypes.byref(bi), ctypes.byref(ex), p32(seq))
+    return {"makespan": ms, "pass_best": pb, "best_index": bi.value, "expanded": ex.value,
+            "sequence": seq if bi.value >= 0 else None}
 
 
 def orc_simulate(inst, seq) -> int:
