@@ -148,7 +148,7 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.ops = b.take<int2>((size_t)s.n * s.m);
   w.beam[0] = b.take<int>((size_t)s.K * s.S);
   w.beam[1] = b.take<int>((size_t)s.K * s.S);
-  w.cbuf = s.single ? b.take<int>(N * s.S) : nullptr;
+  w.cbuf = s.single ? b.take<int>(N * child_stride(s.S)) : nullptr;
   w.keys = b.take<unsigned long long>(std::max(N, (size_t)2 * s.K));
   w.crec[0] = b.take<unsigned long long>(N);
   w.crec[1] = b.take<unsigned long long>(N);

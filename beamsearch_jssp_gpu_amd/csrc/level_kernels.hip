@@ -572,6 +572,109 @@ __device__ __forceinline__ ScoreLevel score_level(const LevelArgs& a, int local)
   return sl;
 }
 
+// Single-phase writer: every kept child of parent p (one wave, G = 64) as a
+// full snapshot in cbuf at row p*B + slot (row stride child_stride(S): whole
+// 128-byte lines; the padding words are written too, so no line is left
+// partly written).  Each lane owns the words o = lane + 64 i; what a child
+// changes in word o depends only on o's role, so the role is resolved once
+// per parent into an id (machine o, 256 + job for job_ready, 512 + job for
+// job_word, 1024 for cmax, 2048 for padding) and the ready position of a job
+// word, and a child then costs a few compares and selects per word.  The
+// kept children's fields are compacted into the wave's LDS rows (finish,
+// cmax, new job word, slot | machine << 8 | position << 16 | job << 24) and
+// read back kWriteUnroll at a time by every lane (broadcast reads), so
+// several children's stores are in flight per wave.
+// NWM: words per lane the caller supports (>= child_stride(S) / 64).
+constexpr int kWriteUnroll = 4;
+
+template <int NWM>
+__device__ __forceinline__ void write_children(const LevelArgs& a, const Layout& Ly, int p, bool pact,
+                                               GL const int* par, const JobView& v, const int (&slot)[2],
+                                               const bool (&keep)[2], int lane, int4* rows) {
+  const int S = Ly.S, B = a.B, m = Ly.m;
+  const int CS = child_stride(S);
+  const bool erase = (a.mode == 1);
+  const int r = v.r;
+  int jlast = -1;
+  if (!erase) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const u64 bm = __ballot(v.rdy[h] && v.pos[h] == r - 1);
+      if (bm) jlast = h * 64 + __ffsll((long long)bm) - 1;
+    }
+  }
+  int pw[NWM], wid[NWM], wpos[NWM];
+#pragma unroll
+  for (int i = 0; i < NWM; ++i) {
+    const int o = lane + 64 * i;
+    pw[i] = (pact && o < S) ? par[o] : 0;
+    int id = o < S ? 1024 : 2048;
+    if (o < Ly.off_jr) id = o;
+    else if (o < Ly.off_jw) id = 256 + (o - Ly.off_jr);
+    else if (o < Ly.off_cmax) id = 512 + (o - Ly.off_jw);
+    wid[i] = id;
+    // erase order: jobs behind the removed position move up one
+    const int ps = jw_pos(pw[i]);
+    wpos[i] = (id >= 512 && id < 1024 && ps != kFinishedPos) ? ps : -1;
+    // swap order: the last ready job takes the removed position
+    if (!erase && id == 512 + jlast) wpos[i] = -2;
+  }
+  // Compact the kept children into the wave's rows.
+  const u64 b0 = __ballot(keep[0]), b1 = __ballot(keep[1]);
+  const int n0 = __popcll(b0), nk = n0 + __popcll(b1);
+  const u64 below = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (keep[h]) {
+      const int rk = (h ? n0 : 0) + __popcll((h ? b1 : b0) & below);
+      const int nx1 = v.nx[h] + 1;
+      const int newjw = nx1 < m ? jw_make(nx1, r - 1, v.nmc[h]) : jw_make(nx1, kFinishedPos, 0);
+      rows[rk] = make_int4(v.f[h], v.cm[h], newjw,
+                           slot[h] | (v.mc[h] << 8) | (v.pos[h] << 16) | ((h * 64 + lane) << 24));
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  GL int* cbase = gp(a.cbuf) + (size_t)p * B * CS;
+  for (int t = 0; t < nk; t += kWriteUnroll) {
+    int4 c[kWriteUnroll];
+#pragma unroll
+    for (int u = 0; u < kWriteUnroll; ++u) c[u] = rows[min(t + u, nk - 1)];
+#pragma unroll
+    for (int u = 0; u < kWriteUnroll; ++u) {
+      if (t + u < nk) {  // wave-uniform
+        const int f = c[u].x, cm = c[u].y, newjw = c[u].z, misc = c[u].w;
+        const int cs = misc & 0xFF, mc = (misc >> 8) & 0xFF, spos = (misc >> 16) & 0xFF;
+        const int j = (int)((uint32_t)misc >> 24);
+        const int idr = 256 + j, idw = 512 + j;
+        GL int* out = cbase + (size_t)cs * CS;
+#pragma unroll
+        for (int i = 0; i < NWM; ++i) {
+          const int o = lane + 64 * i;
+          const int id = wid[i];
+          int w = pw[i];
+          w = (id == mc || id == idr) ? f : w;
+          if (erase) w = (wpos[i] > spos) ? w - 256 : w;
+          else w = (wpos[i] == -2) ? jw_setpos(w, spos) : w;
+          w = (id == idw) ? newjw : w;
+          w = (id == 1024) ? cm : w;
+          if (o < CS) out[o] = w;
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void write_children_any(const LevelArgs& a, const Layout& Ly, int p, bool pact,
+                                                   GL const int* par, const JobView& v, const int (&slot)[2],
+                                                   const bool (&keep)[2], int lane, int4* rows) {
+  const int CS = child_stride(Ly.S);
+  if (CS <= 128) write_children<2>(a, Ly, p, pact, par, v, slot, keep, lane, rows);
+  else if (CS <= 256) write_children<4>(a, Ly, p, pact, par, v, slot, keep, lane, rows);
+  else write_children<kMaxWPL>(a, Ly, p, pact, par, v, slot, keep, lane, rows);
+}
+
 // Score (two-phase) or expand (single-phase) kernel body.  A group of G lanes
 // (G = 16, 32 or 64, G >= n unless G = 64) handles one parent read from the
 // beam in HBM; lane j of the group holds job j (and j + 64 when n > 64).
@@ -579,12 +682,13 @@ __device__ __forceinline__ ScoreLevel score_level(const LevelArgs& a, int local)
 template <bool SINGLE>
 __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_h[kNB];
+  // single-phase: per wave, the kept children of its parent (n <= 128)
+  __shared__ int4 s_rows[SINGLE ? kScoreWaves * 128 : 1];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int nwaves = (int)(blockDim.x >> 6);
   const Layout Ly = make_layout(a.n, a.m);
-  const int n = a.n, m = a.m, S = Ly.S, B = a.B;
+  const int n = a.n, m = a.m, S = Ly.S;
   const int P = a.pcount[a.level];
-  const int NW = (S + 63) >> 6;
   const ScoreLevel sl = score_level(a, local);
   const ScoreDst dst = score_dst(a);
   GL const int2* ops = gp(a.ops);
@@ -634,57 +738,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
     int slot[2], nvalid;
     bool keep[2];
     emit_children(dst, sl, p, pact, G, gl, gmask, lane, v, s_h, acc, slot, keep, nvalid);
-
-    if (SINGLE && !sl.last) {  // G == 64: one parent per wave
-      const int order = (a.mode == 1) ? kOrderErase : kOrderSwap;
-      int pw[kMaxWPL];
-#pragma unroll
-      for (int i = 0; i < kMaxWPL; ++i) {
-        const int o = lane + 64 * i;
-        pw[i] = (pact && i < NW && o < S) ? par[o] : 0;
-      }
-      const int r = v.r;
-      int jlast = -1;
-      if (order == kOrderSwap) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const u64 bm = __ballot(v.rdy[h] && v.pos[h] == r - 1);
-          if (bm) jlast = h * 64 + __ffsll((long long)bm) - 1;
-        }
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        u64 bm = __ballot(keep[h]);
-        while (bm) {
-          const int l = __ffsll((long long)bm) - 1;
-          bm &= bm - 1;
-          ChildMod c;
-          c.j = h * 64 + l;
-          c.mc = __shfl(v.mc[h], l);
-          c.f = __shfl(v.f[h], l);
-          c.cm = __shfl(v.cm[h], l);
-          c.spos = __shfl(v.pos[h], l);
-          c.jlast = jlast;
-          const int cs = __shfl(slot[h], l);
-          const int nx1 = __shfl(v.nx[h], l) + 1;
-          c.newjw = nx1 < m ? jw_make(nx1, r - 1, __shfl(v.nmc[h], l)) : jw_make(nx1, kFinishedPos, 0);
-          GL int* out = gp(a.cbuf) + ((size_t)p * B + cs) * S;
-          if (order == kOrderErase) {
-#pragma unroll
-            for (int i = 0; i < kMaxWPL; ++i) {
-              const int o = lane + 64 * i;
-              if (i < NW && o < S) out[o] = child_word<kOrderErase>(o, pw[i], c, Ly);
-            }
-          } else {
-#pragma unroll
-            for (int i = 0; i < kMaxWPL; ++i) {
-              const int o = lane + 64 * i;
-              if (i < NW && o < S) out[o] = child_word<kOrderSwap>(o, pw[i], c, Ly);
-            }
-          }
-        }
-      }
-    }
+    if (SINGLE && !sl.last) write_children_any(a, Ly, p, pact, par, v, slot, keep, lane, s_rows + wave * 128);
   }
   finish_score_block(acc, s_h, a.bstat, gp(a.hist), local, sl.last);
 }
@@ -1260,7 +1314,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     const u64 rec = crec[x];
     int cm;
     if (single) {
-      GL const int* src = gp(a.cbuf) + (size_t)x * S;
+      GL const int* src = gp(a.cbuf) + (size_t)x * child_stride(S);
       GL int* dst = gp(a.beam_next) + (size_t)q * S;
       for (int o = lane; o < S; o += 64) dst[o] = src[o];
       cm = src[Ly.off_cmax];

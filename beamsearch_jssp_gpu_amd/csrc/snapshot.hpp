@@ -45,6 +45,11 @@ JSSP_HD Layout make_layout(int n, int m) {
   return l;
 }
 
+// Row stride (words) of the single-phase child buffer: S rounded up to whole
+// 128-byte lines, so every child starts on a line and a writer that also
+// fills the padding never leaves a line partly written.
+JSSP_HD int child_stride(int S) { return (S + 31) & ~31; }
+
 JSSP_HD int jw_make(int next_op, int pos, int mc) { return next_op | (pos << 8) | (mc << 16); }
 JSSP_HD int jw_next(int w) { return w & 0xFF; }
 JSSP_HD int jw_pos(int w) { return (w >> 8) & 0xFF; }
