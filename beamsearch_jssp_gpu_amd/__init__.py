@@ -25,7 +25,7 @@ __all__ = [
     "JSSPInstance", "Result", "Engine", "JsspError", "load_library", "library_path",
     "generate_instance", "parse_instance", "resolve_instance_file", "validate_sequence",
     "lower_bound", "beam_search_ich", "beam_search_pms", "run_gpu_beam_search_v1",
-    "run_gpu_beam_search_v2", "default_engine", "Server", "rand_stream",
+    "run_gpu_beam_search_v2", "default_engine", "Server", "rand_stream", "host_solve_pms",
 ]
 
 NO_MAKESPAN = 500000
@@ -74,6 +74,12 @@ class _DriverParams(ctypes.Structure):
                 ("ranks", ctypes.c_int32), ("iterations", ctypes.c_int32), ("time_seed", ctypes.c_uint32)]
 
 
+class _HybridParams(ctypes.Structure):
+    _fields_ = [("gpu_beam_width", ctypes.c_int32), ("accel_factor", ctypes.c_int32), ("phases", ctypes.c_int32),
+                ("threads", ctypes.c_int32), ("ranks", ctypes.c_int32), ("iterations", ctypes.c_int32),
+                ("time_seed", ctypes.c_uint32)]
+
+
 class _ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_slots", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)]
@@ -99,6 +105,13 @@ _SIGNATURES = {
     "jssp_run_passes": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.POINTER(_DriverParams),
                                        ctypes.POINTER(_Result), _I32P]),
     "jssp_rand_stream": (None, [ctypes.c_uint32, ctypes.c_int32, _I32P]),
+    "jssp_host_solve_pms": (ctypes.c_int, [ctypes.POINTER(_Instance), ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_uint32, ctypes.POINTER(_Result)]),
+    "jssp_hybrid_iteration": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_uint32, ctypes.POINTER(_Result)]),
+    "jssp_run_hybrid": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.POINTER(_HybridParams),
+                                       ctypes.POINTER(_Result), _I32P]),
     "jssp_engine_set_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p]),
     "jssp_engine_tuned_lanes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "jssp_engine_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
@@ -302,6 +315,39 @@ class Engine:
                      int(r.levels), float(r.device_ms), int(r.best_seed_index))
         return res, pass_best
 
+    def hybrid_iteration(self, inst: JSSPInstance, gpu_beam_width: int, cpu_beam_width: int, best_known: int,
+                         gpu_seed: int, cpu_seed_base: int, threads: int, phases: int = 1) -> Result:
+        """One hybrid CPU+GPU iteration (hybrid/v1/solver.cu:11-44): the device
+        pass with gpu_seed and threads-1 host BS-PMS passes seeded
+        cpu_seed_base + tid."""
+        seq = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
+        r = _Result()
+        r.sequence = _p32(seq)
+        ci = inst._c()
+        self._check(self.lib.jssp_hybrid_iteration(self.h, ctypes.byref(ci), int(gpu_beam_width),
+                                                   int(cpu_beam_width), int(best_known), int(gpu_seed) & 0xFFFFFFFF,
+                                                   int(phases), int(threads), int(cpu_seed_base) & 0xFFFFFFFF,
+                                                   ctypes.byref(r)))
+        return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                      int(r.levels), float(r.device_ms), int(r.best_seed_index))
+
+    def run_hybrid(self, inst: JSSPInstance, gpu_beam_width: int, accel_factor: int, threads: int, ranks: int,
+                   iterations: int, time_seed: int, phases: int = 1):
+        """The hybrid drivers (hybrid/v1/main.cu:45-85); returns (Result,
+        per-iteration global best)."""
+        seq = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
+        pass_best = np.zeros(max(1, int(iterations)), dtype=np.int32)
+        r = _Result()
+        r.sequence = _p32(seq)
+        ci = inst._c()
+        hp = _HybridParams(int(gpu_beam_width), int(accel_factor), int(phases), int(threads), int(ranks),
+                           int(iterations), int(time_seed) & 0xFFFFFFFF)
+        self._check(self.lib.jssp_run_hybrid(self.h, ctypes.byref(ci), ctypes.byref(hp), ctypes.byref(r),
+                                             _p32(pass_best)))
+        res = Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                     int(r.levels), float(r.device_ms), int(r.best_seed_index))
+        return res, pass_best
+
     def set_autotune(self, on: bool, table_path: Optional[str] = None):
         self._check(self.lib.jssp_engine_set_autotune(self.h, 1 if on else 0,
                                                       table_path.encode() if table_path else None))
@@ -374,6 +420,23 @@ def rand_stream(seed: int, count: int) -> np.ndarray:
     if count > 0:
         load_library().jssp_rand_stream(int(seed) & 0xFFFFFFFF, int(count), _p32(out))
     return out
+
+
+def host_solve_pms(inst: JSSPInstance, beam_width: int, best_known: int = NO_MAKESPAN, seed: int = 0) -> Result:
+    """BS-PMS on the host (cpu/v1_pms/solver.cpp:8-70): srand(seed) then one
+    pass, from a private rand() stream; the CPU side of the hybrid drivers.
+    Needs no device."""
+    lib = load_library()
+    seq = np.zeros(inst.nbjob * inst.nbmachine, dtype=np.int32)
+    r = _Result()
+    r.sequence = _p32(seq)
+    ci = inst._c()
+    rc = lib.jssp_host_solve_pms(ctypes.byref(ci), int(beam_width), int(best_known), int(seed) & 0xFFFFFFFF,
+                                 ctypes.byref(r))
+    if rc != 0:
+        raise JsspError(rc, "host_solve_pms: bad arguments")
+    return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                  int(r.levels), 0.0, 0)
 
 
 def validate_sequence(inst: JSSPInstance, seq: np.ndarray) -> int:

@@ -1,8 +1,11 @@
 // compat.cpp -- reference-signature shims over the C ABI (see
 // include/jssp_compat.hpp for the behaviour contract).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <ctime>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/jssp.h"
@@ -119,4 +122,40 @@ int run_gpu_beam_search_v1(const JSSPInstance& inst, int beam_width, int best_kn
 int run_gpu_beam_search_v2(const JSSPInstance& inst, int beam_width, int best_known, int random_seed,
                            int* out_best_sequence) {
   return run(inst, beam_width, JSSP_MODE_PMS_GPU, 2, best_known, random_seed, 0, out_best_sequence);
+}
+
+namespace {
+
+int hybrid_threads() {
+  const char* env = std::getenv("OMP_NUM_THREADS");
+  const int t = env ? std::atoi(env) : (int)std::thread::hardware_concurrency();
+  return std::max(1, std::min(t, 1024));
+}
+
+int hybrid(const JSSPInstance& inst, int gpu_beam_width, int cpu_beam_width, int best_known, int rank,
+           int iteration, int phases) {
+  jssp_engine* e = default_engine();
+  Flat f = flatten(inst);
+  const uint32_t gpu_seed = (uint32_t)std::rand();  // hybrid/v1/solver.cu:28, thread 0
+  const uint32_t cpu_base = (uint32_t)std::time(nullptr) + (uint32_t)(rank * 1000 + iteration * 100);
+  jssp_result r = {};
+  int rc = jssp_hybrid_iteration(e, &f.in, gpu_beam_width, cpu_beam_width, best_known, gpu_seed, phases,
+                                 hybrid_threads(), cpu_base, &r);
+  if (rc != JSSP_OK) {
+    std::fprintf(stderr, "jssp error %d: %s\n", rc, jssp_last_error(e));
+    std::exit(EXIT_FAILURE);
+  }
+  return r.makespan;
+}
+
+}  // namespace
+
+int run_hybrid_iteration_v1(const JSSPInstance& inst, int gpu_beam_width, int cpu_beam_width, int best_known,
+                            int rank, int iteration) {
+  return hybrid(inst, gpu_beam_width, cpu_beam_width, best_known, rank, iteration, 1);
+}
+
+int run_hybrid_iteration_v2(const JSSPInstance& inst, int gpu_beam_width, int cpu_beam_width, int best_known,
+                            int rank, int iteration) {
+  return hybrid(inst, gpu_beam_width, cpu_beam_width, best_known, rank, iteration, 2);
 }

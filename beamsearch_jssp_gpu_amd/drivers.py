@@ -4,13 +4,17 @@
     python -m beamsearch_jssp_gpu_amd.drivers pms      [instance] [beam_width]            [--ranks R]
     python -m beamsearch_jssp_gpu_amd.drivers gpu_v1   [instance] [beam_width] [iters]    [--ranks R]
     python -m beamsearch_jssp_gpu_amd.drivers gpu_v2   [instance] [beam_width] [iters]    [--ranks R]
+    python -m beamsearch_jssp_gpu_amd.drivers hybrid_v1 | hybrid_v2
+                                  [instance] [gpu_beam_width] [accel_factor] [iters]      [--ranks R --threads T]
     python -m beamsearch_jssp_gpu_amd.drivers bench_pms | bench_gpu_v1 | bench_gpu_v2     [--ranks R]
+    python -m beamsearch_jssp_gpu_amd.drivers bench_hybrid_v1 | bench_hybrid_v2           [--ranks R --threads T]
     python -m beamsearch_jssp_gpu_amd.drivers width_study                                 [--ranks R]
 
 Each command follows one reference program: arguments, defaults, the
 multi-rank pass loop and the printed lines and table columns
 (cpu/v0_ich/main.cpp, cpu/v1_pms/main.cpp, gpu/v1/main.cu,
-gpu/v2_twophases/main.cu, gpu/v1/benchmark.cu, cpu/v1_pms/benchmark.cpp,
+gpu/v2_twophases/main.cu, hybrid/v1/main.cu, hybrid/v2_twophases/main.cu,
+gpu/v1/benchmark.cu, cpu/v1_pms/benchmark.cpp, hybrid/v1/benchmark.cu,
 cpu/profiling/time_width_study.cpp).  `--ranks R` stands for `mpirun -np R`:
 the R ranks run on the one device, rank r seeded srand(time_seed + r).
 `--time-seed` fixes the value the reference reads from time(NULL), so runs
@@ -160,6 +164,70 @@ def cmd_bench(a, solver: str) -> int:
     return 0
 
 
+def hybrid_threads(a) -> int:
+    """OpenMP threads per rank: --threads, else OMP_NUM_THREADS, else the
+    host's thread count (omp_get_max_threads in hybrid/v1/main.cu:54)."""
+    if a.threads:
+        return a.threads
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, int(env)) if env else max(1, os.cpu_count() or 1)
+
+
+def cmd_hybrid(a, phases: int) -> int:
+    """hybrid/v1/main.cu and hybrid/v2_twophases/main.cu: instance,
+    gpu_beam_width (5000), accel_factor (34), iterations (100)."""
+    path = "./la33.txt"
+    if a.instance is not None:
+        path = _resolve(a.instance)
+        if path is None:
+            print(f"Error: cannot locate instance file: {a.instance}", file=sys.stderr)
+            return 1
+    Kg = max(1, a.beam_width) if a.beam_width is not None else 5000
+    accel = max(1, a.iters) if a.iters is not None else 34
+    iters = max(1, a.extra) if a.extra is not None else (a.iterations if a.iterations is not None else 100)
+    Kc = max(1, Kg // accel)
+    T = hybrid_threads(a)
+    inst = parse_instance(path)
+    bar = "=" * 40 if phases == 1 else "=" * 51
+    print("=== Hybrid CPU+GPU Beam Search (v1) ===" if phases == 1 else
+          "=== Hybrid CPU+GPU Beam Search (v2 Two-Phases) ===")
+    print(f"Instance: {path}  |  Jobs: {inst.nbjob}  |  Machines: {inst.nbmachine}")
+    print(f"GPU Beam: {Kg}  |  CPU Beam: {Kc}  |  Accel Factor: {accel}")
+    print(f"MPI Ranks: {a.ranks}  |  GPUs/Node: 1  |  OpenMP Threads/Rank: {T}  |  Iterations: {iters}")
+    print(bar)
+    t0 = time.perf_counter()
+    res, pass_best = default_engine().run_hybrid(inst, Kg, accel, T, a.ranks, iters, a.time_seed, phases=phases)
+    secs = time.perf_counter() - t0
+    for it in range(19, iters, 20):
+        print(f"  iter {it + 1:3d} / {iters:3d} | best makespan = {int(pass_best[it])}")
+    print(bar)
+    print(f"Best Makespan  : {res.makespan}\nExecution Time : {secs:.2f} seconds")
+    return 0
+
+
+def cmd_bench_hybrid(a, phases: int) -> int:
+    """hybrid/v1/benchmark.cu, hybrid/v2_twophases/benchmark.cu: the Lawrence
+    table, 50 iterations, accel factor 34."""
+    iters = a.iterations if a.iterations is not None else 50
+    T = hybrid_threads(a)
+    ids = a.only.split(",") if a.only else LAWRENCE_IDS
+    print(_header("Kbest"))
+    for idx, iid in enumerate(LAWRENCE_IDS):
+        if iid not in ids:
+            continue
+        path = _resolve(iid)
+        if path is None:
+            print(f"SKIP la{iid}  (file not found)")
+            continue
+        inst = parse_instance(path)
+        Kg = lawrence_beam_width(idx)
+        t0 = time.perf_counter()
+        res, _ = default_engine().run_hybrid(inst, Kg, 34, T, a.ranks, iters, a.time_seed, phases=phases)
+        print(_row(Kg, path, inst, res.makespan, time.perf_counter() - t0), flush=True)
+    print("\nAll Lawrence instances finished.")
+    return 0
+
+
 def cmd_width_study(a) -> int:
     """cpu/profiling/time_width_study.cpp: PMS, 100 iterations per width,
     rank r of width w seeded srand(time + r + w)."""
@@ -184,11 +252,14 @@ def cmd_width_study(a) -> int:
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m beamsearch_jssp_gpu_amd.drivers", description=__doc__,
                                  formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("command", choices=["ich", "pms", "gpu_v1", "gpu_v2", "bench_pms", "bench_gpu_v1",
-                                        "bench_gpu_v2", "width_study"])
+    ap.add_argument("command", choices=["ich", "pms", "gpu_v1", "gpu_v2", "hybrid_v1", "hybrid_v2", "bench_pms",
+                                        "bench_gpu_v1", "bench_gpu_v2", "bench_hybrid_v1", "bench_hybrid_v2",
+                                        "width_study"])
     ap.add_argument("instance", nargs="?")
     ap.add_argument("beam_width", nargs="?", type=int)
-    ap.add_argument("iters", nargs="?", type=int)
+    ap.add_argument("iters", nargs="?", type=int, help="gpu_*: iterations; hybrid_*: accel factor")
+    ap.add_argument("extra", nargs="?", type=int, help="hybrid_*: iterations")
+    ap.add_argument("--threads", type=int, default=0, help="hybrid_*: threads per rank (OpenMP threads)")
     ap.add_argument("--ranks", type=int, default=1, help="MPI world size replayed on the device")
     ap.add_argument("--time-seed", type=int, default=None, help="value of time(NULL) (default: now)")
     ap.add_argument("--iterations", type=int, default=None, help="passes (default: the program's own)")
@@ -198,7 +269,7 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if a.time_seed is None:
         a.time_seed = int(time.time())
-    if a.iters is not None and a.iterations is None:
+    if a.iters is not None and a.iterations is None and not a.command.startswith("hybrid"):
         a.iterations = a.iters
     if a.ranks < 1:
         print("Error: --ranks must be >= 1", file=sys.stderr)
@@ -214,6 +285,10 @@ def main(argv=None) -> int:
         return cmd_pms(a)
     if a.command in ("gpu_v1", "gpu_v2"):
         return cmd_gpu(a, 1 if a.command == "gpu_v1" else 2)
+    if a.command in ("hybrid_v1", "hybrid_v2"):
+        return cmd_hybrid(a, 1 if a.command == "hybrid_v1" else 2)
+    if a.command in ("bench_hybrid_v1", "bench_hybrid_v2"):
+        return cmd_bench_hybrid(a, 1 if a.command == "bench_hybrid_v1" else 2)
     if a.command.startswith("bench_"):
         return cmd_bench(a, a.command[len("bench_"):])
     return cmd_width_study(a)

@@ -14,8 +14,11 @@
  *   run_gpu_beam_search_v2  gpu/v2_twophases/solver.cuh:6-12 -> jssp_solve, mode PMS_GPU, phases 2
  *   parse_instance          cpu/common/instance_parser.h:9   -> jssp_parse_instance_file
  *   resolve_instance_file   cpu/common/instance_parser.h:6   -> jssp_resolve_instance_file
- *   drivers' multi-pass loop + MPI_Allreduce(MIN)  gpu/v1/main.cu:60-71,
- *                           cpu/v1_pms/main.cpp:56-73  -> jssp_solve with n_seeds/n_passes
+ *   drivers' multi-pass loop + MPI_Allreduce(MIN)  gpu/v1/main.cu:55-71,
+ *                           cpu/v1_pms/main.cpp:55-73  -> jssp_run_passes (also
+ *                           jssp_solve_seeded, jssp_solve with n_seeds/n_passes)
+ *   run_hybrid_iteration_v1/_v2  hybrid/v1/solver.cuh, hybrid/v2_twophases/solver.cuh
+ *                           -> jssp_hybrid_iteration; hybrid drivers -> jssp_run_hybrid
  * C++ callers that want the reference's exact function signatures include
  * jssp_compat.hpp instead.
  */
@@ -166,6 +169,50 @@ int jssp_run_passes(jssp_engine* engine, const jssp_instance* inst, const jssp_d
 /* out[i] = the i-th value of rand() after srand(seed) (glibc's generator),
  * computed without touching the process-wide rand() state. */
 void jssp_rand_stream(uint32_t seed, int32_t count, int32_t* out);
+
+/* BS-PMS on the host (cpu/v1_pms/solver.cpp:8-70; the CPU side of the hybrid
+ * drivers), one pass: srand(seed) then beam_search_pms(inst, K, best_known),
+ * from a private rand() stream.  Needs no engine or device.  result as
+ * PMS_EXACT (makespan = best complete cmax; sequence filled when found and
+ * result->sequence is set; device_ms = 0). */
+int jssp_host_solve_pms(const jssp_instance* inst, int32_t beam_width, int32_t best_known, uint32_t seed,
+                        jssp_result* result);
+
+/* One hybrid CPU+GPU iteration (hybrid/v1/solver.cu:11-44,
+ * hybrid/v2_twophases/solver.cu:11-44): the device pass (PMS_GPU, beam
+ * gpu_beam_width, seed gpu_seed; phases 1 = gpu/v1, 2 = gpu/v2) and
+ * threads - 1 host BS-PMS passes (beam cpu_beam_width, thread tid drawing
+ * srand(cpu_seed_base + tid), tid = 1 .. threads - 1), all pruning with
+ * best_known, run at the same time.  result->makespan = min(best_known,
+ * every pass); best_seed_index = the thread that reached it first (0 = the
+ * device pass, then threads in order); sequence filled when a pass improved
+ * on best_known. */
+int jssp_hybrid_iteration(jssp_engine* engine, const jssp_instance* inst, int32_t gpu_beam_width,
+                          int32_t cpu_beam_width, int32_t best_known, uint32_t gpu_seed, int32_t phases,
+                          int32_t threads, uint32_t cpu_seed_base, jssp_result* result);
+
+/* The hybrid drivers (hybrid/v1/main.cu:45-85, hybrid/v2_twophases/main.cu,
+ * and the per-instance loops of hybrid/v1/benchmark.cu and
+ * hybrid/v2_twophases/benchmark.cu): rank r seeds
+ * srand(time_seed + r); each iteration every rank runs jssp_hybrid_iteration
+ * with gpu_seed = its next rand(), cpu_beam_width = max(1, gpu_beam_width /
+ * accel_factor), cpu_seed_base = time_seed + r*1000 + iteration*100 and
+ * best_known = the minimum over all ranks of the iterations before
+ * (MPI_Allreduce MIN, from JSSP_NO_MAKESPAN).  The ranks' device passes of an
+ * iteration run as one batch, their host passes on a thread pool beside it.
+ * result / pass_best as jssp_run_passes; best_seed_index = (iteration *
+ * ranks + rank) * threads + tid. */
+typedef struct jssp_hybrid_params {
+  int32_t gpu_beam_width;
+  int32_t accel_factor;  /* >= 1 (the reference's default is 34) */
+  int32_t phases;        /* 1 = hybrid/v1, 2 = hybrid/v2_twophases */
+  int32_t threads;       /* threads per rank (OpenMP threads): 1 device + threads-1 host passes */
+  int32_t ranks;
+  int32_t iterations;
+  uint32_t time_seed;
+} jssp_hybrid_params;
+int jssp_run_hybrid(jssp_engine* engine, const jssp_instance* inst, const jssp_hybrid_params* params,
+                    jssp_result* result, int32_t* pass_best);
 
 /* Batched solve: advances all `count` instances together, one level per
  * step; a step is three launches (score, filter, rank) whose grids cover every

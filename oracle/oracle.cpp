@@ -38,6 +38,10 @@
 //                   srand(time_seed) (itself pinned by the goldens); the
 //                   reference's own beam_search_pms run under the same
 //                   per-rank streams (oracle/_ref) gives the same values.
+//   orc_hybrid_driver  hybrid/v1/main.cu:45-85 + hybrid/v1/solver.cu:11-44
+//                   over orc_glibc_rand streams.  Pinned: one thread reduces
+//                   to orc_gpu_driver; each host pass is orc_pms_exact after
+//                   srand(its seed) (pinned by the goldens).
 //   orc_gpu_driver  gpu/v1/main.cu:55-71 over orc_glibc_rand streams and
 //                   orc_gpu_solve.  Pinned: one rank and one pass reduce to
 //                   orc_gpu_solve with seed = the first rand() after
@@ -544,6 +548,50 @@ int orc_gpu_driver(int n, int m, const int* mach, const int* dur, int beam_width
     if (pass_best) pass_best[it] = global;
   }
   if (best_index) *best_index = bi;
+  if (expanded) *expanded = ex;
+  return global;
+}
+
+// The hybrid driver loop, hybrid/v1/main.cu:45-85 with the iteration of
+// hybrid/v1/solver.cu:11-44: rank r seeds srand(time_seed + r); each
+// iteration, thread 0 draws seed = rand() and runs the PMS_GPU search with
+// beam gpu_k, threads tid = 1 .. threads-1 run beam_search_pms with beam
+// cpu_k from srand(time_seed + r*1000 + it*100 + tid), all with best_known =
+// global_best; iteration best = min(best_known, every result); local_best =
+// min(local_best, iteration best); global_best = MPI_Allreduce(MIN).  Every
+// thread draws from its own generator (in the reference the threads share
+// one, and race).  pass_best[it] = global_best after iteration it; returns
+// global_best.
+int orc_hybrid_driver(int n, int m, const int* mach, const int* dur, int gpu_k, int cpu_k, int threads, int ranks,
+                      int iterations, unsigned time_seed, int* pass_best, long long* expanded) {
+  std::vector<GlibcRand> streams;
+  for (int r = 0; r < ranks; ++r) streams.emplace_back(time_seed + (unsigned)r);
+  std::vector<int> local(ranks, kNoMakespan), seq((size_t)n * m);
+  int global = kNoMakespan;
+  long long ex = 0;
+  for (int it = 0; it < iterations; ++it) {
+    for (int r = 0; r < ranks; ++r) {
+      int best_iter = global;
+      const unsigned seed = (unsigned)streams[r].next();
+      long long e1 = 0;
+      int found = 0;
+      int v = orc_gpu_solve(n, m, mach, dur, gpu_k, 2, global, seed, seq.data(), &e1, &found, nullptr, nullptr);
+      ex += e1;
+      best_iter = std::min(best_iter, v);
+      for (int tid = 1; tid < threads; ++tid) {
+        GlibcRand g(time_seed + (unsigned)(r * 1000 + it * 100 + tid));
+        long long e2 = 0;
+        v = pms_pass(n, m, mach, dur, cpu_k, global, [&g] { return g.next(); }, &e2);
+        ex += e2;
+        best_iter = std::min(best_iter, v);
+      }
+      if (best_iter < local[r]) local[r] = best_iter;
+    }
+    int red = kNoMakespan;
+    for (int r = 0; r < ranks; ++r) red = std::min(red, local[r]);
+    global = red;
+    if (pass_best) pass_best[it] = global;
+  }
   if (expanded) *expanded = ex;
   return global;
 }

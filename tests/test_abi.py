@@ -40,6 +40,8 @@ def test_compat_symbols_exported():
                 "beam_search_pms(JSSPInstance const&, int, int)",
                 "run_gpu_beam_search_v1(JSSPInstance const&, int, int, int, int*)",
                 "run_gpu_beam_search_v2(JSSPInstance const&, int, int, int, int*)",
+                "run_hybrid_iteration_v1(JSSPInstance const&, int, int, int, int, int)",
+                "run_hybrid_iteration_v2(JSSPInstance const&, int, int, int, int, int)",
                 "parse_instance(char const*, JSSPInstance&)",
                 "resolve_instance_file(char const*, char*, int)"]:
         assert sig in out, sig

@@ -124,3 +124,60 @@ def test_lawrence_runner_rows_equal_oracle(engine, tmp_path, capsys):
             else:
                 want = orc_gpu_driver(insts[name], 1000, 2, iters, 20240601)["makespan"]
             assert int(r[5]) == want, (cmd, name)
+
+
+@pytest.mark.parametrize("phases", [1, 2])
+def test_hybrid_driver_equals_oracle_loop(engine, phases):
+    """jssp_run_hybrid (hybrid/v1/main.cu loop): device passes batched per
+    iteration, host BS-PMS passes on threads; equal to the oracle's loop."""
+    from testlib import orc_hybrid_driver
+    inst = splitmix_instance(41, 12, 6, 1, 99)
+    Kg, accel, T, R, E, t = 64, 8, 3, 2, 3, 1700000123
+    print("hybrid driver", phases, flush=True)
+    got, pass_best = engine.run_hybrid(_b(inst), Kg, accel, T, R, E, t, phases=phases)
+    want = orc_hybrid_driver(inst, Kg, Kg // accel, T, R, E, t)
+    assert got.makespan == want["makespan"]
+    np.testing.assert_array_equal(pass_best, want["pass_best"])
+    assert got.expanded_states == want["expanded"]
+    assert got.found
+    assert bs.validate_sequence(_b(inst), got.sequence) == got.makespan
+
+
+def test_hybrid_iteration_is_min_of_its_passes(engine):
+    inst = splitmix_instance(43, 10, 5, 1, 99)
+    Kg, Kc, T, bk, gseed, cbase = 32, 4, 4, bs.NO_MAKESPAN, 987654, 42000
+    got = engine.hybrid_iteration(_b(inst), Kg, Kc, bk, gseed, cbase, T, phases=2)
+    g = engine.solve(_b(inst), Kg, mode=bs.MODE_PMS_GPU, phases=2, best_known=bk, seed=gseed)
+    hs = [bs.host_solve_pms(_b(inst), Kc, bk, cbase + tid) for tid in range(1, T)]
+    parts = [g.makespan] + [h.makespan for h in hs]
+    assert got.makespan == min([bk] + parts)
+    assert got.expanded_states == g.expanded_states + sum(h.expanded_states for h in hs)
+    first = [i for i, v in enumerate(parts) if v == got.makespan][0]
+    assert got.best_seed_index == first
+    assert bs.validate_sequence(_b(inst), got.sequence) == got.makespan
+
+
+def test_hybrid_cli_equals_oracle(engine, tmp_path, capsys):
+    """python -m beamsearch_jssp_gpu_amd.drivers hybrid_v2 la01 64 8 2
+    --threads 3 (hybrid/v2_twophases/main.cu arguments): the printed best
+    makespan equals the oracle's hybrid loop."""
+    import os
+    from beamsearch_jssp_gpu_amd import drivers
+    from testlib import load_json, orc_hybrid_driver
+    d = load_json("lawrence.json")["la01"]
+    with open(tmp_path / "la01.txt", "w") as f:
+        f.write(f"{d['n']} {d['m']}\n")
+        for j in range(d["n"]):
+            f.write(" ".join(f"{d['machine'][i]} {d['duration'][i]}" for i in range(j * d["m"], (j + 1) * d["m"]))
+                    + "\n")
+    cwd = os.getcwd()
+    try:
+        assert drivers.main(["hybrid_v2", "01", "64", "8", "2", "--dir", str(tmp_path), "--threads", "3",
+                             "--time-seed", "9"]) == 0
+    finally:
+        os.chdir(cwd)
+    out = capsys.readouterr().out
+    line = [l for l in out.splitlines() if l.startswith("Best Makespan")][0]
+    want = orc_hybrid_driver(golden_instances()["la01"], 64, 8, 3, 1, 2, 9)["makespan"]
+    assert int(line.split(":")[1]) == want
+    assert "GPU Beam: 64  |  CPU Beam: 8  |  Accel Factor: 8" in out

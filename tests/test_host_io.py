@@ -130,3 +130,37 @@ def test_lawrence_runner_skips_missing_files(tmp_path, capsys):
     skips = [l for l in out if l.startswith("SKIP")]
     assert skips[0] == "SKIP la01  (file not found)" and len(skips) == 40
     assert [drivers.lawrence_beam_width(i) for i in (0, 9, 10, 39)] == [1000, 1000, 5000, 5000]
+
+
+def test_host_pms_matches_reference_goldens():
+    """The host BS-PMS (the CPU side of the hybrid drivers) gives the compiled
+    reference's makespans (tests/golden/golden_ref.json), the oracle's
+    expanded counts, and sequences that re-simulate to the makespan."""
+    from testlib import golden_instances, orc_pms
+    g = load_json("golden_ref.json")
+    insts = golden_instances()
+    cases = [c for c in g["cases"] if c["solver"] == "pms"]
+    assert len(cases) > 50
+    for c in cases:
+        inst = insts[c["instance"]]
+        b = bs.JSSPInstance(inst.machine, inst.duration)
+        r = bs.host_solve_pms(b, c["K"], c["best_known"], c["seed"])
+        assert r.makespan == c["makespan"], c
+        ms, ex = orc_pms(inst, c["K"], c["best_known"], c["seed"])
+        assert r.expanded_states == ex, c
+        if r.found:
+            assert orc_simulate(inst, r.sequence) == r.makespan, c
+
+
+def test_host_pms_random_shapes_match_oracle():
+    from testlib import orc_pms, random_instance
+    rng = np.random.default_rng(99)
+    for t in range(40):
+        n, m = int(rng.integers(1, 14)), int(rng.integers(1, 8))
+        inst = random_instance(rng, n, m, perm=bool(t % 2))
+        K = int(rng.choice([1, 3, 16, 100]))
+        seed = int(rng.integers(0, 2**32))
+        bk = int(rng.choice([500000, 150, 400]))
+        r = bs.host_solve_pms(bs.JSSPInstance(inst.machine, inst.duration), K, bk, seed)
+        ms, ex = orc_pms(inst, K, bk, seed)
+        assert (r.makespan, r.expanded_states) == (ms, ex), (n, m, K, seed, bk)

@@ -21,8 +21,8 @@ import numpy as np
 import pytest
 
 from testlib import (Inst, golden_instances, libc_rand, load_json, lower_bound, oracle, orc_bruteforce,
-                     orc_gpu, orc_gpu_driver, orc_ich, orc_pms, orc_pms_driver, orc_rand, orc_simulate, p32,
-                     random_instance, ref_lib, splitmix_instance, valid_trail_entries)
+                     orc_gpu, orc_gpu_driver, orc_hybrid_driver, orc_ich, orc_pms, orc_pms_driver, orc_rand,
+                     orc_simulate, p32, random_instance, ref_lib, splitmix_instance, valid_trail_entries)
 
 ICH_DET, PMS_GPU = 1, 2
 NO = 500000
@@ -263,3 +263,27 @@ def test_gpu_driver_reduces_to_single_solves():
         bound = min([bound] + [x["makespan"] for x in res])
         mins.append(bound)
     assert d["expanded"] == exp and list(d["pass_best"]) == mins and d["makespan"] == bound
+
+
+def test_hybrid_driver_reduces_to_its_parts():
+    """One thread per rank is the GPU driver; with host threads, every
+    iteration's best is the minimum of the GPU pass and the PMS passes from
+    their seeds (each pinned on its own)."""
+    inst = splitmix_instance(17, 8, 5, 1, 99)
+    a = orc_hybrid_driver(inst, 16, 4, 1, 2, 3, 555)
+    b = orc_gpu_driver(inst, 16, 2, 3, 555)
+    assert a["makespan"] == b["makespan"] and list(a["pass_best"]) == list(b["pass_best"])
+    assert a["expanded"] == b["expanded"]
+    T, t = 3, 777
+    h = orc_hybrid_driver(inst, 16, 4, T, 1, 2, t)
+    seeds = orc_rand(t, 2)
+    bound, exp, mins = NO, 0, []
+    for it in range(2):
+        w = orc_gpu(inst, 16, PMS_GPU, bound, int(seeds[it]), trace=False)
+        best, exp = min(bound, w["makespan"]), exp + w["expanded"]
+        for tid in range(1, T):
+            ms, ex = orc_pms(inst, 4, bound, t + it * 100 + tid)
+            best, exp = min(best, ms), exp + ex
+        bound = min(bound, best)
+        mins.append(bound)
+    assert list(h["pass_best"]) == mins and h["expanded"] == exp

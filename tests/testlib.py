@@ -53,6 +53,8 @@ def oracle():
                                        ctypes.c_int, ctypes.c_uint, I32P, ip, ll]
         lib.orc_gpu_driver.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_uint, I32P, ip, ll, I32P]
+        lib.orc_hybrid_driver.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint, I32P, ll]
         _oracle = lib
     return _oracle
 
@@ -199,6 +201,15 @@ def orc_gpu_driver(inst: Inst, K: int, ranks: int, iterations: int, time_seed: i
                                  time_seed & 0xFFFFFFFF, p32(pb), ctypes.byref(bi), ctypes.byref(ex), p32(seq))
     return {"makespan": ms, "pass_best": pb, "best_index": bi.value, "expanded": ex.value,
             "sequence": seq if bi.value >= 0 else None}
+
+
+def orc_hybrid_driver(inst: Inst, gpu_k: int, cpu_k: int, threads: int, ranks: int, iterations: int,
+                      time_seed: int):
+    pb = np.zeros(iterations, np.int32)
+    ex = ctypes.c_longlong()
+    ms = oracle().orc_hybrid_driver(inst.n, inst.m, p32(inst.machine), p32(inst.duration), gpu_k, cpu_k, threads,
+                                    ranks, iterations, time_seed & 0xFFFFFFFF, p32(pb), ctypes.byref(ex))
+    return {"makespan": ms, "pass_best": pb, "expanded": ex.value}
 
 
 def orc_simulate(inst, seq) -> int:
