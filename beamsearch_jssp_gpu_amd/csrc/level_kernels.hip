@@ -1414,7 +1414,8 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
 // ---------------------------------------------------------------------------
 // Best complete child and its sequence, by walking the trail backwards.
 __device__ void backtrack_body(const LevelArgs& a) {
-  __shared__ int s_cnt[128];
+  __shared__ uint8_t s_job[128 * 255];  // one job per level (n <= 128, m <= 255)
+  __shared__ int s_pj[2];
   __shared__ uint32_t s_e[4];
   __shared__ u64 s_b[4];
   int p = a.bt_parent, j = a.bt_job;
@@ -1431,38 +1432,46 @@ __device__ void backtrack_body(const LevelArgs& a) {
     best = wave_min64(best);
     if (lane == 0) { s_e[wave] = ex; s_b[wave] = best; }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    ex = 0; best = ~0ull;
-    for (int w = 0; w < 4; ++w) { ex += s_e[w]; best = s_b[w] < best ? s_b[w] : best; }
-    *a.expanded += (u64)ex;
-    a.lvl_exp[L - 1] = ex;
-    if (best == ~0ull) {
-      a.result[0] = a.best_known; a.result[1] = 0; a.result[2] = 0;
-      return;
+    if (threadIdx.x == 0) {
+      ex = 0; best = ~0ull;
+      for (int w = 0; w < 4; ++w) { ex += s_e[w]; best = s_b[w] < best ? s_b[w] : best; }
+      *a.expanded += (u64)ex;
+      a.lvl_exp[L - 1] = ex;
+      s_pj[0] = -1;
+      if (best == ~0ull) {
+        a.result[0] = a.best_known; a.result[1] = 0; a.result[2] = 0;
+      } else {
+        const uint32_t x = (uint32_t)best;
+        const int cmax = (int)(best >> 32);
+        s_pj[0] = (int)(x / (uint32_t)a.B);
+        s_pj[1] = rec_job(a.crec[x]);
+        a.result[0] = min(cmax, a.best_known);
+        a.result[1] = 1;
+        a.result[2] = cmax;
+      }
     }
-    const uint32_t x = (uint32_t)best;
-    const int cmax = (int)(best >> 32);
-    p = (int)(x / (uint32_t)a.B);
-    j = rec_job(a.crec[x]);
-    a.result[0] = min(cmax, a.best_known);
-    a.result[1] = 1;
-    a.result[2] = cmax;
-  } else if (threadIdx.x != 0) {
-    return;
+    __syncthreads();
+    p = s_pj[0];
+    j = s_pj[1];
+    if (p < 0) return;  // no complete child (block-uniform)
   }
+  // Thread 0 walks the trail back into LDS (one job byte per level), then
+  // thread j numbers job j's operations in sequence order.
+  if (threadIdx.x == 0) {
+    s_job[L - 1] = (uint8_t)j;
+    int r = p;
+    for (int l = L - 2; l >= 0; --l) {
+      const int t = a.trail[(size_t)l * a.K + r];
+      s_job[l] = (uint8_t)(t & 0xFF);
+      r = t >> 8;
+    }
+  }
+  __syncthreads();
   int* seq = a.seq_out;
-  seq[L - 1] = j;
-  int r = p;
-  for (int l = L - 2; l >= 0; --l) {
-    const int t = a.trail[(size_t)l * a.K + r];
-    seq[l] = t & 0xFF;
-    r = t >> 8;
-  }
-  for (int i = 0; i < a.n; ++i) s_cnt[i] = 0;
-  for (int l = 0; l < L; ++l) {
-    const int jj = seq[l];
-    seq[l] = jj * m + s_cnt[jj] + 1;
-    s_cnt[jj]++;
+  for (int jj = threadIdx.x; jj < a.n; jj += blockDim.x) {
+    int k = 0;
+    for (int l = 0; l < L; ++l)
+      if (s_job[l] == jj) seq[l] = jj * m + (k++) + 1;
   }
 }
 
