@@ -70,9 +70,9 @@ HostPmsResult host_pms(const jssp_instance& in, int K, int best_known, RandStrea
   std::vector<Handle> heap;
   heap.reserve(K + 1);
   std::vector<int> free_slots;
-  // Trail for the sequence: per level, (parent index in the previous level's
-  // order, job) of each kept child.
-  std::vector<std::vector<std::pair<int, int>>> trail;
+  // Trail for the sequence: per level, (parent position in the previous
+  // level's order) << 8 | job of each kept child.
+  std::vector<std::vector<uint32_t>> trail;
   int best_pos = -1, best_job = -1;
   for (int step = 1; step <= L + 10 && ncur > 0; ++step) {
     heap.clear();
@@ -138,7 +138,8 @@ HostPmsResult host_pms(const jssp_instance& in, int K, int best_known, RandStrea
     const int cnt = (int)heap.size();
     if (want_sequence) {
       trail.emplace_back(cnt);
-      for (int i = 0; i < cnt; ++i) trail.back()[i] = {nxt.parent[heap[i].idx], nxt.job[heap[i].idx]};
+      for (int i = 0; i < cnt; ++i)
+        trail.back()[i] = ((uint32_t)nxt.parent[heap[i].idx] << 8) | (uint32_t)nxt.job[heap[i].idx];
     }
     order.resize(cnt);
     for (int i = 0; i < cnt; ++i) order[i] = heap[i].idx;
@@ -154,9 +155,9 @@ HostPmsResult host_pms(const jssp_instance& in, int K, int best_known, RandStrea
     jobs[L - 1] = best_job;
     int pos = best_pos;
     for (int st = L - 1; st >= 1; --st) {
-      const std::pair<int, int>& t = trail[st - 1][pos];
-      jobs[st - 1] = t.second;
-      pos = t.first;
+      const uint32_t t = trail[st - 1][pos];
+      jobs[st - 1] = (int)(t & 0xFFu);
+      pos = (int)(t >> 8);
     }
     std::vector<int> next_op(n, 0);
     res.sequence.resize(L);
