@@ -164,6 +164,7 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.sel[1] = b.take<SelState>(1);
   w.pcount = b.take<int>(s.L + 1);
   w.trail = b.take<int>((size_t)std::max(1, s.L - 1) * s.K);
+  w.jump = s.exact ? nullptr : b.take<uint16_t>((size_t)((std::max(1, s.L - 1) + 31) / 32) * s.K);
   w.expanded = b.take<unsigned long long>(1);
   w.lvl_exp = b.take<uint32_t>(s.L);
   w.sel_log = b.take<SelState>(s.L);
@@ -364,6 +365,7 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   rc = upload_ops(e, in, w.ops);
   if (rc) return rc;
   InstDesc desc = make_desc(s, w, &q, in);
+  desc.base.jump = w.jump;  // this solve runs k_jump_b before its backtrack
   if (e->profiling) desc.base.sel_log = w.sel_log;
   // The descriptor goes up through the pinned staging buffer; the step
   // kernels run with no block map (one instance, local block = grid block).
@@ -386,6 +388,7 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
     JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, rank_variant(s, &q), e->stream,
                                  prof.pair(kKindRank, &ep, l < s.L - 1)));
   }
+  JSSP_HIP(e, launch_step_jump(d_desc, s.K, s.L, e->stream));
   JSSP_HIP(e, launch_step_backtrack(d_desc, nullptr, 1, e->stream));
   JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
   int* h_res = reinterpret_cast<int*>(e->hbuf);

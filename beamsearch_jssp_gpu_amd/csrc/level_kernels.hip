@@ -1415,6 +1415,7 @@ __global__ __launch_bounds__(256) void k_exact_materialize(LevelArgs a, int coun
 // Best complete child and its sequence, by walking the trail backwards.
 __device__ void backtrack_body(const LevelArgs& a) {
   __shared__ uint8_t s_job[128 * 255];  // one job per level (n <= 128, m <= 255)
+  __shared__ int s_seg[(128 * 255 + 31) / 32];  // path index at the top of each 32-level group
   __shared__ int s_pj[2];
   __shared__ uint32_t s_e[4];
   __shared__ u64 s_b[4];
@@ -1455,9 +1456,31 @@ __device__ void backtrack_body(const LevelArgs& a) {
     j = s_pj[1];
     if (p < 0) return;  // no complete child (block-uniform)
   }
-  // Thread 0 walks the trail back into LDS (one job byte per level), then
-  // thread j numbers job j's operations in sequence order.
-  if (threadIdx.x == 0) {
+  // The trail is walked back into LDS (one job byte per level), then thread
+  // j numbers job j's operations in sequence order.  With the jump table,
+  // thread 0 first finds the path's index at the top of every 32-level group
+  // (one load per group) and the groups are walked in parallel, one thread
+  // each; without it (batches, exact modes) thread 0 walks every level.
+  if (a.jump) {
+    if (threadIdx.x == 0) {
+      s_job[L - 1] = (uint8_t)j;
+      int idx = p;
+      for (int g = (L - 2) >> 5; g >= 0; --g) {
+        s_seg[g] = idx;
+        idx = a.jump[(size_t)g * a.K + idx];
+      }
+    }
+    __syncthreads();
+    const int groups = L >= 2 ? ((L - 2) >> 5) + 1 : 0;
+    for (int g = threadIdx.x; g < groups; g += blockDim.x) {
+      int idx = s_seg[g];
+      for (int l = min(32 * g + 31, L - 2); l >= 32 * g; --l) {
+        const int t = a.trail[(size_t)l * a.K + idx];
+        s_job[l] = (uint8_t)(t & 0xFF);
+        idx = t >> 8;
+      }
+    }
+  } else if (threadIdx.x == 0) {
     s_job[L - 1] = (uint8_t)j;
     int r = p;
     for (int l = L - 2; l >= 0; --l) {
@@ -1633,6 +1656,26 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_fused_b(InstDesc* desc
   rank_body<true, MODE, SMALL>(a, d.crec[(level + 1) & 1], r.y, d.nblk_rank);
 }
 
+// Jump table of a single solve (base.jump): thread (g, q) follows survivor
+// q after the last level of 32-level group g (level L-2 for the last group)
+// down the trail to the group's first level and records its ancestor after
+// level 32 g - 1.  All groups and survivors at once, so the backtrack's walk
+// becomes one load per group plus 32 levels per group in parallel.
+__global__ __launch_bounds__(256) void k_jump_b(InstDesc* desc) {
+  const LevelArgs& b = desc[0].base;
+  const int L = b.L, K = b.K;
+  if (!b.jump || L < 2) return;
+  const int groups = ((L - 2) >> 5) + 1;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)groups * K) return;
+  const int g = (int)(t / K), q = (int)(t % K);
+  const int top = min(32 * g + 31, L - 2);
+  if (q >= b.pcount[top + 1]) return;  // survivors after level top
+  int idx = q;
+  for (int l = top; l >= 32 * g; --l) idx = b.trail[(size_t)l * K + idx] >> 8;
+  b.jump[(size_t)g * K + q] = (uint16_t)idx;
+}
+
 __global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map) {
   InstDesc& d = desc[block_ref(map).x];
   if (d.lvl_filter < d.base.L || d.bt_done) return;  // lvl_filter reaches L after the last rank
@@ -1697,6 +1740,13 @@ hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_
 
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s) {
   hipLaunchKernelGGL(k_backtrack_b, dim3(nblk), dim3(256), 0, s, desc, map);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_jump(InstDesc* desc, int K, int L, hipStream_t s) {
+  if (L < 2) return hipSuccess;
+  const long long threads = (long long)(((L - 2) >> 5) + 1) * K;
+  hipLaunchKernelGGL(k_jump_b, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, desc);
   return hipGetLastError();
 }
 

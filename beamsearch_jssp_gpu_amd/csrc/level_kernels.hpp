@@ -82,6 +82,8 @@ struct LevelArgs {
   SelState* sel_log;         // L entries or null: copy of each level's selection state
   int* pcount;               // L+1 parent counts
   int* trail;                // (L-1) x K
+  uint16_t* jump;            // null, or ceil((L-1)/32) x K: for survivor q after the last level of
+                             // 32-level group g, its ancestor after level 32 g - 1 (k_jump_b)
   unsigned long long* expanded;  // running total
   uint32_t* lvl_exp;         // L: children evaluated per level
   int* seq_out;              // L
@@ -158,6 +160,8 @@ enum RankVariant : int {
 hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
                             hipStream_t s, const EventPair* ev = nullptr);
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
+// Single solve: fills base.jump (when set) for the backtrack; K, L of the solve.
+hipError_t launch_step_jump(InstDesc* desc, int K, int L, hipStream_t s);
 
 // Exact (host-replay) modes, driven level by level from the host.
 hipError_t launch_init(const LevelArgs& a, hipStream_t s);

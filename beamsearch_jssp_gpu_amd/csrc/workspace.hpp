@@ -27,6 +27,7 @@ struct Work {
   SelState* sel[2];
   int* pcount;
   int* trail;
+  uint16_t* jump;
   unsigned long long* expanded;
   uint32_t* lvl_exp;
   SelState* sel_log;
