@@ -974,7 +974,15 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
 
   // Every load of the prologue goes out at once: the histogram copies, one
-  // score-block partial per thread (nblk_score <= 512), the level's parents.
+  // score-block partial per thread (nblk_score <= 512), the level's parents,
+  // and this block's first tile of keys and digits (read within the buffers'
+  // K*B capacity; slots past this level's count are ignored below).
+  GL const u64* keys = gp(a.keys);
+  GL const uint16_t* kdig = gp(a.kdig);
+  const uint32_t cap = (uint32_t)a.K * (uint32_t)a.B;
+  const uint32_t xf = (uint32_t)local * blockDim.x + tid;
+  const u64 key_first = xf < cap ? keys[xf] : ~0ull;
+  const uint32_t dg_first = xf < cap ? (uint32_t)kdig[xf] : (uint32_t)kNB;
   uint32_t h0 = 0, h1 = 0;
 #pragma unroll
   for (int c = 0; c < kHistCopies; ++c) {
@@ -1038,16 +1046,16 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     sel->sorted_from = single ? cum_lt : Kp;
   }
 
-  GL const u64* keys = gp(a.keys);
-  GL const uint16_t* kdig = gp(a.kdig);
   GL u64* cand = gp(a.cand);
   GL u64* tiles_keys = gp(a.ref_buf);
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
-    // Both loads at once: the digit of an invalid slot is never read.
-    const u64 key = inb ? keys[x] : ~0ull;
-    const uint32_t dg = inb ? (uint32_t)kdig[x] : (uint32_t)kNB;
+    const bool first = base == (uint32_t)local * blockDim.x;
+    // Both loads at once (the first tile's were issued in the prologue): the
+    // digit of an invalid slot is never read.
+    const u64 key = inb ? (first ? key_first : keys[x]) : ~0ull;
+    const uint32_t dg = inb ? (first ? dg_first : (uint32_t)kdig[x]) : (uint32_t)kNB;
     const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;
     const bool tie = d == T1;
