@@ -164,3 +164,37 @@ def test_host_pms_random_shapes_match_oracle():
         r = bs.host_solve_pms(bs.JSSPInstance(inst.machine, inst.duration), K, bk, seed)
         ms, ex = orc_pms(inst, K, bk, seed)
         assert (r.makespan, r.expanded_states) == (ms, ex), (n, m, K, seed, bk)
+
+
+REF_TREE = "/root/reference"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REF_TREE, "lawrence_instances")),
+                    reason="the reference tree (instance files) is not present")
+def test_parser_matches_reference_parser_on_every_instance_file():
+    """All 120 instance files the reference ships (40 Lawrence, 80 Taillard)
+    parse to the same n, m, machines and durations with this parser and with
+    the reference's own parse_instance (compiled into oracle/_ref)."""
+    import ctypes
+    from testlib import ref_lib
+    ref = ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    ref.ref_parse.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.c_int]
+    files = []
+    for d in ("lawrence_instances", "taillard_instances"):
+        files += sorted(os.path.join(REF_TREE, d, f) for f in os.listdir(os.path.join(REF_TREE, d)))
+    assert len(files) == 120
+    for path in files:
+        inst = bs.parse_instance(path)
+        cap = inst.nbjob * inst.nbmachine
+        n, m = ctypes.c_int(), ctypes.c_int()
+        mach = np.zeros(cap, np.int32)
+        dur = np.zeros(cap, np.int32)
+        got = ref.ref_parse(path.encode(), ctypes.byref(n), ctypes.byref(m),
+                            mach.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                            dur.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), cap)
+        assert got == cap and (n.value, m.value) == (inst.nbjob, inst.nbmachine), path
+        np.testing.assert_array_equal(inst.machine.ravel(), mach, err_msg=path)
+        np.testing.assert_array_equal(inst.duration.ravel(), dur, err_msg=path)
