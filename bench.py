@@ -301,8 +301,8 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one MI355X: this build runs on one GPU (independent solves would be spread "
-              "one per GPU); --gpus must be 1", file=sys.stderr)
+        print("bench.py measures one MI355X: the engine drives one GPU per process (independent solves "
+              "would be spread one per GPU); --gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
     import beamsearch_jssp_gpu_amd as bs
