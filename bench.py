@@ -20,6 +20,8 @@ Also reported on the same JSON line:
                    as one batch, vs the same 16 solves one after another
   batched          256 uneven instances in one batched solve vs the plain
                    one-instance-at-a-time loop (same results checked)
+  small_configs    10x5 K=64 BS-ICH (ICH_EXACT) over 32 instances and 20x20
+                   K=1024 single-phase PMS_GPU / ICH_DET over 16 instances
   serving          Poisson arrivals into the continuous-batching server vs the
                    plain loop on the same request list: solves/s, p50/p99;
                    then every request queued at once (the server's capacity
@@ -258,6 +260,48 @@ def cpu_baseline(inst, eng, bs, K=2048, seed=12345):
             "cpu_model": _cpu_model()}
 
 
+def small_configs(eng, bs):
+    """The two small configurations on the GPU: BS-ICH on 10x5 (durations
+    U[5,99], instance seeds 1..32, K=64; ICH_EXACT, the reference's own
+    rand()/heap semantics) and single-phase PMS_GPU / ICH_DET on 20x20
+    (U[1,99], seeds 1..16, K=1024).  Wall time per solve includes the upload
+    and readback."""
+    out = {}
+    for name, (n, m, dmin, dmax, seeds, K, mode, phases) in {
+            "ich_exact_10x5_k64": (10, 5, 5, 99, 32, 64, bs.MODE_ICH_EXACT, 2),
+            "pms_gpu_20x20_k1024_single_phase": (20, 20, 1, 99, 16, 1024, bs.MODE_PMS_GPU, 1),
+            "ich_det_20x20_k1024_single_phase": (20, 20, 1, 99, 16, 1024, bs.MODE_ICH_DET, 1)}.items():
+        insts = [bs.generate_instance(sd, n, m, dmin, dmax) for sd in range(1, seeds + 1)]
+        for inst in insts[:2]:
+            eng.solve(inst, K, mode=mode, phases=phases, seed=12345)
+        t0 = time.perf_counter()
+        res = [eng.solve(inst, K, mode=mode, phases=phases, seed=12345) for inst in insts]
+        dt = time.perf_counter() - t0
+        states = sum(r.expanded_states for r in res)
+        out[name] = {"instances": seeds, "ms_per_solve": round(dt / seeds * 1e3, 3),
+                     "device_ms_per_solve": round(sum(r.device_ms for r in res) / seeds, 3),
+                     "states_per_s": round(states / dt, 1), "makespans": [r.makespan for r in res]}
+    return out
+
+
+def cpu_small_config(bs):
+    """The reference's own BS-ICH (oracle/_ref) on the 10x5 configuration, one
+    thread: the CPU side of small_configs' first entry."""
+    ref_so = os.path.join(REPO, "oracle", "_ref", "libjssp_ref.so")
+    if not os.path.exists(ref_so):
+        return None
+    lib = ctypes.CDLL(ref_so)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    lib.ref_ich.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_uint]
+    insts = [bs.generate_instance(sd, 10, 5, 5, 99) for sd in range(1, 33)]
+    t0 = time.perf_counter()
+    ms = [lib.ref_ich(10, 5, i.machine.ctypes.data_as(I32P), i.duration.ctypes.data_as(I32P), 64, 12345)
+          for i in insts]
+    dt = time.perf_counter() - t0
+    return {"workload": "BS-ICH 10x5, durations U[5,99], instance seeds 1..32, K=64, srand(12345)",
+            "kind": "reference", "cores": 1, "ms_per_solve": round(dt / 32 * 1e3, 3), "makespans": ms}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -365,9 +409,17 @@ def main():
         line["batched"] = batched(eng, bs, args.batch)
         log(f"serving: {args.requests} Poisson requests")
         line["serving"] = serving(eng, bs, args.requests, args.budget_gib, args.serve_load)
+        log("small configurations: 10x5 K=64 BS-ICH, 20x20 K=1024 single-phase")
+        line["small_configs"] = small_configs(eng, bs)
     if not args.no_cpu:
         log("CPU baseline (reference BS-ICH, one thread)")
         line["cpu_baseline"] = cpu_baseline(inst, eng, bs, K=args.cpu_beam, seed=seed)
+        small = cpu_small_config(bs)
+        if small is not None:
+            if "small_configs" in line:
+                gpu_ms = line["small_configs"]["ich_exact_10x5_k64"]["makespans"]
+                small["equal_makespans"] = small["makespans"] == gpu_ms
+            line["cpu_baseline"]["small_config"] = small
     eng.close()
     print(json.dumps(line), flush=True)
 
