@@ -181,3 +181,21 @@ def test_hybrid_cli_equals_oracle(engine, tmp_path, capsys):
     want = orc_hybrid_driver(golden_instances()["la01"], 64, 8, 3, 1, 2, 9)["makespan"]
     assert int(line.split(":")[1]) == want
     assert "GPU Beam: 64  |  CPU Beam: 8  |  Accel Factor: 8" in out
+
+
+def test_host_pms_equals_device_pms_exact(engine):
+    """The host BS-PMS (hybrid workers) and the device-expanded PMS_EXACT mode
+    replay the same reference search: same makespan, expanded count and
+    sequence, including edge shapes (one job, one machine) and a tight bound."""
+    from testlib import random_instance
+    rng = np.random.default_rng(5)
+    cases = [(1, 5), (6, 1), (8, 4), (12, 7)]
+    for t, (n, m) in enumerate(cases):
+        inst = random_instance(rng, n, m)
+        b = _b(inst)
+        for K, bk, seed in ((1, bs.NO_MAKESPAN, 3), (16, bs.NO_MAKESPAN, 77 + t), (64, 0, 5)):
+            h = bs.host_solve_pms(b, K, bk, seed)
+            d = engine.solve(b, K, mode=bs.MODE_PMS_EXACT, best_known=bk, seed=seed)
+            assert (h.makespan, h.found, h.expanded_states) == (d.makespan, d.found, d.expanded_states), (n, m, K)
+            if h.found:
+                np.testing.assert_array_equal(h.sequence, d.sequence)
