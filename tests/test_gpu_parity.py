@@ -196,3 +196,28 @@ def test_reference_signature_shims(engine):
     assert bs.run_gpu_beam_search_v2(b, 32, bs.NO_MAKESPAN, 77) == ms
     assert bs.beam_search_ich(b, 16, seed=12345) == orc_ich(inst, 16, 12345)[0]
     assert bs.beam_search_pms(b, 16, bs.NO_MAKESPAN, seed=5) == orc_pms(inst, 16, bs.NO_MAKESPAN, 5)[0]
+
+
+def test_limits_and_argument_errors(engine):
+    """The largest shapes the engine accepts run and re-simulate (255
+    machines; 65536-wide beam; 128 jobs); one past any limit is an error code,
+    not a crash."""
+    rng = np.random.default_rng(11)
+    inst = random_instance(rng, 3, 255)                       # 765 levels, 255 machines
+    r = engine.solve(_binst(inst), 8, mode=bs.MODE_ICH_DET, seed=1)
+    w = orc_gpu(inst, 8, bs.MODE_ICH_DET, bs.NO_MAKESPAN, 1, trace=False)
+    assert r.makespan == w["makespan"] and r.expanded_states == w["expanded"]
+    inst = random_instance(rng, 12, 3)                        # K = 65536 over 36 levels
+    for mode in (bs.MODE_ICH_DET, bs.MODE_PMS_GPU):
+        r = engine.solve(_binst(inst), 65536, mode=mode, seed=2)
+        w = orc_gpu(inst, 65536, mode, bs.NO_MAKESPAN, 2, trace=False)
+        assert r.makespan == w["makespan"] and r.expanded_states == w["expanded"]
+        np.testing.assert_array_equal(r.sequence, w["sequence"])
+    too_big = [(random_instance(rng, 129, 2), 4), (random_instance(rng, 2, 256), 4),
+               (random_instance(rng, 4, 4), 65537)]
+    for bad, K in too_big:
+        with pytest.raises(bs.JsspError) as e:
+            engine.solve(_binst(bad), K, mode=bs.MODE_ICH_DET)
+        assert e.value.code == 2  # JSSP_ERR_LIMIT
+    with pytest.raises(bs.JsspError):
+        engine.solve(_binst(random_instance(rng, 4, 4)), 0, mode=bs.MODE_ICH_DET)
