@@ -679,6 +679,32 @@ __device__ __forceinline__ void write_children_any(const LevelArgs& a, const Lay
 // (G = 16, 32 or 64, G >= n unless G = 64) handles one parent read from the
 // beam in HBM; lane j of the group holds job j (and j + 64 when n > 64).
 // Mode 1 = ICH_DET (every ready op, slot = ready position), mode 2 = PMS_GPU.
+// A parent's cmax and, per lane, the job word and the predecessor's finish of
+// its jobs gl and 64 + gl (lanes beyond n, or an absent parent: zeros).
+struct ParentWords {
+  int pc;
+  int w[2], pred[2];
+};
+
+__device__ __forceinline__ ParentWords load_parent_words(const LevelArgs& a, const Layout& Ly, int p, bool ok,
+                                                         int gl, int NH) {
+  ParentWords r;
+  GL const int* par = gp(a.beam_cur) + (size_t)(ok ? p : 0) * Ly.S;
+  r.pc = ok ? par[Ly.off_cmax] : 0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    r.w[h] = 0;
+    r.pred[h] = 0;
+    const int j = h * 64 + gl;
+    if (ok && h < NH && j < Ly.n) {
+      r.w[h] = par[Ly.off_jw + j];
+      // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
+      r.pred[h] = (Ly.m == 1) ? (j > 0 ? par[Ly.off_jr + j - 1] : 0) : par[Ly.off_jr + j];
+    }
+  }
+  return r;
+}
+
 template <bool SINGLE>
 __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_h[kNB];
@@ -688,26 +714,31 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   const int nwaves = (int)(blockDim.x >> 6);
   const Layout Ly = make_layout(a.n, a.m);
   const int n = a.n, m = a.m, S = Ly.S;
-  const int P = a.pcount[a.level];
-  const ScoreLevel sl = score_level(a, local);
-  const ScoreDst dst = score_dst(a);
-  GL const int2* ops = gp(a.ops);
-  for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
-  __syncthreads();
-
   const int G = a.G;
   const int lg = (G == 64) ? 6 : (G == 32 ? 5 : 4);
   const int PPW = 64 >> lg;
   const int sub = lane >> lg, gl = lane & (G - 1);
   const u64 gmask = (G == 64) ? ~0ull : (((1ull << G) - 1ull) << (sub * G));
   const int NH = (G == 64) ? ((n + 63) >> 6) : 1;
+  const int pb0 = (local * nwaves + wave) * PPW;
+  // The wave's first parent row is loaded before the level's parent count
+  // arrives (every row below K is allocated; the words are used only when
+  // the parent exists).
+  ParentWords first = load_parent_words(a, Ly, pb0 + sub, pb0 + sub < a.K, gl, NH);
+  const int P = a.pcount[a.level];
+  const ScoreLevel sl = score_level(a, local);
+  const ScoreDst dst = score_dst(a);
+  GL const int2* ops = gp(a.ops);
+  for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
+  __syncthreads();
   ScoreAcc acc;
 
-  for (int pb = (local * nwaves + wave) * PPW; pb < P; pb += nblk * nwaves * PPW) {
+  for (int pb = pb0; pb < P; pb += nblk * nwaves * PPW) {
     const int p = pb + sub;
     const bool pact = p < P;
     GL const int* par = gp(a.beam_cur) + (size_t)(pact ? p : 0) * S;
-    const int pc = pact ? par[Ly.off_cmax] : 0;
+    const ParentWords pw = (pb == pb0) ? first : load_parent_words(a, Ly, p, pact, gl, NH);
+    const int pc = pact ? pw.pc : 0;
     JobView v;
     v.r = 0;
 #pragma unroll
@@ -715,9 +746,8 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
       v.rdy[h] = false; v.pos[h] = kFinishedPos; v.mc[h] = 0; v.f[h] = 0; v.cm[h] = 0; v.nx[h] = 0; v.nmc[h] = 0;
       const int j = h * 64 + gl;
       if (pact && h < NH && j < n) {
-        const int w = par[Ly.off_jw + j];
-        // gpu_snapshot.cuh:104 -- with M == 1 the predecessor is op-1 of the previous job.
-        const int pred = (m == 1) ? (j > 0 ? par[Ly.off_jr + j - 1] : 0) : par[Ly.off_jr + j];
+        const int w = pw.w[h];
+        const int pred = pw.pred[h];
         v.pos[h] = jw_pos(w);
         if (v.pos[h] != kFinishedPos) {
           v.rdy[h] = true;
@@ -1517,18 +1547,43 @@ __device__ __forceinline__ BlockRef block_ref(const BlockRef* map) {
   return map ? map[blockIdx.x] : make_int2(0, (int)blockIdx.x);
 }
 
+// The descriptor of an instance, read with one coalesced load per wave (lane
+// i holds its bytes [8i, 8i + 8)) and broadcast word by word: one memory
+// round trip, where field-by-field scalar loads take several dependent ones
+// (the level-indexed fields wait for the level counter).  Every lane of the
+// wave must be active.  Fields the kernel stores are written through the
+// descriptor pointer, never through the copy.
+__device__ __forceinline__ InstDesc fetch_desc(const InstDesc* dp) {
+  static_assert(sizeof(InstDesc) % 8 == 0 && sizeof(InstDesc) <= 64 * 8, "one load per lane");
+  constexpr int NW = (int)(sizeof(InstDesc) / 8);
+  const int lane = threadIdx.x & 63;
+  GL const u64* src = reinterpret_cast<GL const u64*>(gp(dp));
+  const u64 w = lane < NW ? src[lane] : 0ull;
+  uint32_t words[2 * NW];
+#pragma unroll
+  for (int i = 0; i < NW; ++i) {
+    words[2 * i] = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)w, i);
+    words[2 * i + 1] = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(w >> 32), i);
+  }
+  InstDesc d;
+  __builtin_memcpy(&d, words, sizeof(d));
+  return d;
+}
+
 __device__ __forceinline__ LevelArgs resolve(const InstDesc& d, int level) {
   LevelArgs a = d.base;
   a.level = level;
   a.lseed = d.base.lseed + (uint32_t)level;
-  a.beam_cur = d.beam[level & 1];
-  a.beam_next = d.beam[(level + 1) & 1];
-  a.hist = d.hist[level & 1];
-  a.hist_next = d.hist[(level + 1) & 1];
-  a.sel = d.sel[level & 1];
-  a.sel_prev = d.sel[(level + 1) & 1];
-  a.sel_next = d.sel[(level + 1) & 1];
-  a.crec = d.crec[level & 1];
+  // selects, not indexing: the descriptor may be a register copy
+  const bool odd = (level & 1) != 0;
+  a.beam_cur = odd ? d.beam[1] : d.beam[0];
+  a.beam_next = odd ? d.beam[0] : d.beam[1];
+  a.hist = odd ? d.hist[1] : d.hist[0];
+  a.hist_next = odd ? d.hist[0] : d.hist[1];
+  a.sel = odd ? d.sel[1] : d.sel[0];
+  a.sel_prev = odd ? d.sel[0] : d.sel[1];
+  a.sel_next = odd ? d.sel[0] : d.sel[1];
+  a.crec = odd ? d.crec[1] : d.crec[0];
   return a;
 }
 
@@ -1601,8 +1656,8 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
 
 // Advances a level counter (block 0 of an instance, after its reader kernel
 // has completed).
-__device__ __forceinline__ void advance(int& counter, int local, int L) {
-  if (local == 0 && threadIdx.x == 0 && counter < L) counter = counter + 1;
+__device__ __forceinline__ void advance(int& counter, int cur, int local, int L) {
+  if (local == 0 && threadIdx.x == 0 && cur < L) counter = cur + 1;
 }
 
 // Two-phase score: keys and selection digits only.
@@ -1610,10 +1665,12 @@ __global__ __launch_bounds__(1024) void k_score_b(InstDesc* desc, const BlockRef
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_score;
-  advance(d.lvl_rank, r.y, d.base.L);
-  if (level >= d.base.L) return;
+  // descriptor fields are read before the counter store (scalar loads)
   const LevelArgs a = resolve(d, level);
-  score_body<false>(a, r.y, d.nblk_score);
+  const int nblk = d.nblk_score;
+  advance(d.lvl_rank, d.lvl_rank, r.y, a.L);
+  if (level >= a.L) return;
+  score_body<false>(a, r.y, nblk);
 }
 
 // Single-phase expand: the same plus every kept child's snapshot.
@@ -1621,18 +1678,20 @@ __global__ __launch_bounds__(1024) void k_expand_b(InstDesc* desc, const BlockRe
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   const int level = d.lvl_score;
-  advance(d.lvl_rank, r.y, d.base.L);
-  if (level >= d.base.L) return;
+  // descriptor fields are read before the counter store (scalar loads)
   const LevelArgs a = resolve(d, level);
-  score_body<true>(a, r.y, d.nblk_score);
+  const int nblk = d.nblk_score;
+  advance(d.lvl_rank, d.lvl_rank, r.y, a.L);
+  if (level >= a.L) return;
+  score_body<true>(a, r.y, nblk);
 }
 
 __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
-  InstDesc& d = desc[r.x];
+  const InstDesc d = fetch_desc(desc + r.x);
   const int level = d.lvl_filter;
-  if (d.fused) advance(d.lvl_rank, r.y, d.base.L);
-  else advance(d.lvl_score, r.y, d.base.L);
+  if (d.fused) advance(desc[r.x].lvl_rank, d.lvl_rank, r.y, d.base.L);
+  else advance(desc[r.x].lvl_score, d.lvl_score, r.y, d.base.L);
   if (level >= d.base.L - 1) return;  // the last level only records the best child
   const LevelArgs a = resolve(d, level);
   filter_body(a, r.y, d.nblk_filter);
@@ -1645,10 +1704,12 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, cons
   InstDesc& d = desc[r.x];
   if (d.fused) return;
   const int level = d.lvl_rank;
-  advance(d.lvl_filter, r.y, d.base.L);
-  if (level >= d.base.L - 1) return;
+  // descriptor fields are read before the counter store (scalar loads)
   const LevelArgs a = resolve(d, level);
-  rank_body<false, 0>(a, nullptr, r.y, d.nblk_rank);
+  const int nblk = d.nblk_rank;
+  advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
+  if (level >= a.L - 1) return;
+  rank_body<false, 0>(a, nullptr, r.y, nblk);
 }
 
 // Rank + materialise + score of the next level (fused form), per mode.
@@ -1656,12 +1717,17 @@ template <int MODE, bool SMALL>
 __global__ __launch_bounds__(64 * kRankWaves) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
-  if (!d.fused || d.base.mode != MODE || (d.base.S <= 128 && d.base.n <= 64) != SMALL) return;
+  // one round of loads for the variant test (no short-circuit)
+  const bool small = (d.base.S <= 128) & (d.base.n <= 64);
+  if (!((d.fused != 0) & (d.base.mode == MODE) & (small == SMALL))) return;
   const int level = d.lvl_rank;
-  advance(d.lvl_filter, r.y, d.base.L);
-  if (level >= d.base.L - 1) return;
+  // descriptor fields are read before the counter store (scalar loads)
   const LevelArgs a = resolve(d, level);
-  rank_body<true, MODE, SMALL>(a, d.crec[(level + 1) & 1], r.y, d.nblk_rank);
+  u64* const crec_next = (level & 1) ? d.crec[0] : d.crec[1];
+  const int nblk = d.nblk_rank;
+  advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
+  if (level >= a.L - 1) return;
+  rank_body<true, MODE, SMALL>(a, crec_next, r.y, nblk);
 }
 
 // Jump table of a single solve (base.jump): thread (g, q) follows survivor
