@@ -1750,9 +1750,15 @@ __global__ __launch_bounds__(256) void k_jump_b(InstDesc* desc) {
   b.jump[(size_t)g * K + q] = (uint16_t)idx;
 }
 
-__global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map) {
-  InstDesc& d = desc[block_ref(map).x];
-  if (d.lvl_filter < d.base.L || d.bt_done) return;  // lvl_filter reaches L after the last rank
+// status (optional): status[instance] = whether the instance is finished
+// after this launch (host memory the server reads after the launch's event).
+__global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const BlockRef* map, int* status) {
+  const int inst = block_ref(map).x;
+  InstDesc& d = desc[inst];
+  if (d.lvl_filter < d.base.L || d.bt_done) {  // lvl_filter reaches L after the last rank
+    if (status && threadIdx.x == 0) status[inst] = d.bt_done;
+    return;
+  }
   LevelArgs a = resolve(d, d.base.L - 1);
   a.bt_parent = -1;
   a.bt_job = -1;
@@ -1760,6 +1766,7 @@ __global__ __launch_bounds__(256) void k_backtrack_b(InstDesc* desc, const Block
   if (threadIdx.x == 0) {
     d.bt_done = 1;
     if (d.shared_bound && a.result[1]) atomicMin(d.shared_bound, a.result[2]);
+    if (status) status[inst] = 1;
   }
 }
 }  // namespace
@@ -1812,8 +1819,8 @@ hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_
   return hipGetLastError();
 }
 
-hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s) {
-  hipLaunchKernelGGL(k_backtrack_b, dim3(nblk), dim3(256), 0, s, desc, map);
+hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s, int* status) {
+  hipLaunchKernelGGL(k_backtrack_b, dim3(nblk), dim3(256), 0, s, desc, map, status);
   return hipGetLastError();
 }
 

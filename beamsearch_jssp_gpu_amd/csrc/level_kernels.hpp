@@ -159,7 +159,10 @@ enum RankVariant : int {
 };
 hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
                             hipStream_t s, const EventPair* ev = nullptr);
-hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s);
+// status (optional, device-visible host memory): per instance of the map,
+// whether it is finished after this launch.
+hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s,
+                                 int* status = nullptr);
 // Single solve: fills base.jump (when set) for the backtrack; K, L of the solve.
 hipError_t launch_step_jump(InstDesc* desc, int K, int L, hipStream_t s);
 

@@ -73,6 +73,20 @@ struct jssp_server {
   int n_variant[jssp::kRankVariants] = {};  // resident requests per rank kernel variant
   size_t flds = 0, rlds = 0;
   std::vector<InstDesc> hdesc;
+  // Pipelined steps: each step's last kernel writes every resident slot's
+  // finished flag into pinned host memory (status[parity], no copy command)
+  // and the step ends with an event; the next serve step reads that status
+  // after enqueuing its own kernels, so the device always has the current
+  // step queued while the host harvests.  status_ticket records which request
+  // held each slot when the step was enqueued (a slot freed and re-admitted
+  // since then is skipped).
+  int* status[2] = {nullptr, nullptr};
+  int* status_dev[2] = {nullptr, nullptr};
+  std::vector<uint64_t> status_ticket[2];
+  hipEvent_t status_ev[2] = {nullptr, nullptr};
+  int status_pending = -1;  // parity of the step whose status is not read yet
+  int parity = 0;
+  hipStream_t copy_stream = nullptr;  // result reads of finished requests, beside the running step
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -230,25 +244,27 @@ int rebuild_maps(jssp_server* s) {
   return JSSP_OK;
 }
 
-// Collects finished requests and frees their slots.
-int harvest(jssp_server* s) {
+// Collects the requests that the status of step `par` shows finished and
+// frees their slots.  Their results are final (every later kernel returns at
+// once for a finished request), so they are read on the copy stream while the
+// step in flight keeps running.
+int harvest(jssp_server* s, int par) {
   jssp_engine* e = s->e;
-  std::vector<InstDesc> ds((size_t)s->nslots);
-  JSSP_HIP(e, hipMemcpyAsync(ds.data(), s->d_desc, (size_t)s->nslots * sizeof(InstDesc), hipMemcpyDeviceToHost,
-                             e->stream));
-  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  JSSP_HIP(e, hipEventSynchronize(s->status_ev[par]));
+  const volatile int* done = s->status[par];
   for (int i = 0; i < s->nslots; ++i) {
-    if (!s->slot_ticket[i] || !ds[i].bt_done) continue;
-    Request& r = s->reqs[s->slot_ticket[i]];
+    const uint64_t t = s->slot_ticket[i];
+    if (!t || t != s->status_ticket[par][i] || !done[i]) continue;
+    Request& r = s->reqs[t];
     const Work& w = s->slot_work[i];
     int res[4];
     unsigned long long ex = 0;
     r.seq.assign((size_t)s->slot_shape[i].L, 0);
-    JSSP_HIP(e, hipMemcpyAsync(res, w.result, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    JSSP_HIP(e, hipMemcpyAsync(&ex, w.expanded, sizeof(ex), hipMemcpyDeviceToHost, e->stream));
+    JSSP_HIP(e, hipMemcpyAsync(res, w.result, 3 * sizeof(int), hipMemcpyDeviceToHost, s->copy_stream));
+    JSSP_HIP(e, hipMemcpyAsync(&ex, w.expanded, sizeof(ex), hipMemcpyDeviceToHost, s->copy_stream));
     JSSP_HIP(e, hipMemcpyAsync(r.seq.data(), w.seq, r.seq.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                               e->stream));
-    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+                               s->copy_stream));
+    JSSP_HIP(e, hipStreamSynchronize(s->copy_stream));
     r.makespan = res[0];
     r.found = res[1];
     r.expanded = (int64_t)ex;
@@ -304,6 +320,21 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->slot_work.resize(s->nslots);
   s->slot_region.resize(s->nslots);
   s->hdesc.resize(s->nslots);
+  for (int k = 0; k < 2; ++k) {
+    s->status_ticket[k].assign(s->nslots, 0);
+    if (hipHostMalloc((void**)&s->status[k], (size_t)s->nslots * sizeof(int)) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&s->status_dev[k], s->status[k], 0) != hipSuccess ||
+        hipEventCreateWithFlags(&s->status_ev[k], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      jssp_server_destroy(s);
+      return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot allocate the step status buffers");
+    }
+  }
+  if (hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    (void)hipGetLastError();
+    jssp_server_destroy(s);
+    return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot create the copy stream");
+  }
   *out = s;
   return JSSP_OK;
 }
@@ -312,6 +343,14 @@ void jssp_server_destroy(jssp_server* s) {
   if (!s) return;
   std::lock_guard<std::recursive_mutex> lk(s->e->mu);
   (void)hipStreamSynchronize(s->e->stream);
+  if (s->copy_stream) {
+    (void)hipStreamSynchronize(s->copy_stream);
+    (void)hipStreamDestroy(s->copy_stream);
+  }
+  for (int k = 0; k < 2; ++k) {
+    if (s->status_ev[k]) (void)hipEventDestroy(s->status_ev[k]);
+    if (s->status[k]) (void)hipHostFree(s->status[k]);
+  }
   if (s->arena) (void)hipFree(s->arena);
   if (s->meta) (void)hipFree(s->meta);
   delete s;
@@ -361,6 +400,8 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
     rc = rebuild_maps(s);
     if (rc) return rc;
   }
+  const int prev = s->status_pending;
+  s->status_pending = -1;
   if (s->nact > 0) {
     for (int t = 0; t < levels; ++t) {
       if (s->nms > 0) JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, false, e->stream));
@@ -369,8 +410,24 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
         if (s->n_variant[v] > 0)
           JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr, s->nmr, s->rlds, v, e->stream));
     }
-    JSSP_HIP(e, jssp::launch_step_backtrack(s->d_desc, s->d_slots, s->nact, e->stream));
-    rc = harvest(s);
+    // This step's status, read by the next serve step (or by this one when
+    // it runs no levels).
+    const int par = s->parity;
+    s->parity ^= 1;
+    s->status_ticket[par] = s->slot_ticket;
+    JSSP_HIP(e, jssp::launch_step_backtrack(s->d_desc, s->d_slots, s->nact, e->stream, s->status_dev[par]));
+    JSSP_HIP(e, hipEventRecord(s->status_ev[par], e->stream));
+    s->status_pending = par;
+  }
+  if (prev >= 0) {
+    rc = harvest(s, prev);
+    if (rc) return rc;
+  }
+  if (levels == 0 && s->status_pending >= 0) {
+    // A step of no levels reads everything enqueued so far.
+    const int cur = s->status_pending;
+    s->status_pending = -1;
+    rc = harvest(s, cur);
     if (rc) return rc;
   }
   if (active) {
@@ -407,8 +464,9 @@ int jssp_serve_cancel(jssp_server* s, uint64_t ticket) {
   Request& r = it->second;
   if (r.state == kDone || r.state == kCancelled) return JSSP_OK;
   if (r.state == kRunning) {
-    // Every enqueued step has completed (serve_step synchronises), so the
-    // slot can be dropped from the maps and its workspace released.
+    // A step may still be running on the slot's workspace: wait for it, then
+    // drop the slot from the maps and release its workspace.
+    JSSP_HIP(s->e, hipStreamSynchronize(s->e->stream));
     arena_free(s, s->slot_region[r.slot]);
     s->slot_ticket[r.slot] = 0;
     s->maps_dirty = true;

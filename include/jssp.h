@@ -268,7 +268,11 @@ void jssp_server_destroy(jssp_server* server);
 int jssp_serve_submit(jssp_server* server, const jssp_instance* inst, const jssp_params* params,
                       uint64_t* ticket);
 /* Advances the device by up to `levels` level steps, admitting queued work as
- * slots and budget free up.  Returns the number of active requests in *active. */
+ * slots and budget free up.  Returns the number of active requests in *active.
+ * Steps are pipelined: the call returns with its step still running on the
+ * device and collects the requests that the previous step finished, so a
+ * request that finishes during a step is reported by the following call (a
+ * call with levels == 0 waits for everything enqueued and collects it). */
 int jssp_serve_step(jssp_server* server, int32_t levels, int32_t* active);
 /* JSSP_OK and the result when done, JSSP_ERR_PENDING otherwise.  The result's
  * sequence pointer is caller-owned as in jssp_solve. */
