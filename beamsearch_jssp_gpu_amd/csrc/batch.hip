@@ -1,11 +1,16 @@
 // batch.hip -- many solves advanced together, one level per step.
 //
 // Every instance of a batch has its own workspace and descriptor; a step is
-// three launches (score, filter, rank) whose grids are the concatenation of
-// every instance's blocks (block map entry = instance, local block).  The
-// level lives on the device, so a step is the same launch sequence whatever
-// the instances' levels; finished instances' blocks return at once.  Steps
-// are captured once in a hipGraph and replayed.  The reference solves one
+// one launch per kernel kind (score when some instance scores separately,
+// filter, and the rank kernel of each variant present) whose grid is the
+// concatenation of its instances' blocks (block map entry = instance, local
+// block).  The level lives on the device, so a step is the same launch
+// sequence whatever the instances' levels.  An instance of L levels is done
+// after L steps, which the host knows in advance: the batch runs in segments
+// that end where some instance ends, and each segment's maps hold only the
+// instances still running, so no launch dispatches blocks of finished
+// instances.  A segment's steps are captured in a hipGraph (cached by launch
+// layout) and replayed.  The reference solves one
 // instance per call and one call per MPI rank (gpu/v1/main.cu:60-71); the
 // multi-seed driver below runs its seeds as one batch instead, sharing the
 // best bound through a device atomic.
@@ -25,10 +30,13 @@
 
 namespace jssp {
 
+void release_step_graphs(jssp_engine* e) {
+  for (auto& g : e->step_graphs) (void)hipGraphExecDestroy(g.second);
+  e->step_graphs.clear();
+}
+
 void release_batch_state(jssp_engine* e) {
-  if (e->step_graph) (void)hipGraphExecDestroy(e->step_graph);
-  e->step_graph = nullptr;
-  e->step_graph_key.clear();
+  release_step_graphs(e);
   if (e->bbuf) (void)hipFree(e->bbuf);
   e->bbuf = nullptr;
   e->bcap = 0;
@@ -37,6 +45,18 @@ void release_batch_state(jssp_engine* e) {
 namespace {
 
 constexpr int kStepsPerGraph = 16;
+constexpr size_t kGraphCacheMax = 64;
+
+// Steps [first, first + steps) of a batch and the blocks of the instances
+// still running during them, per kernel kind.
+struct Segment {
+  int first = 0, steps = 0;
+  int nms = 0, nmf = 0;
+  BlockRef *d_ms = nullptr, *d_mf = nullptr;
+  int nmr[kRankVariants] = {};
+  BlockRef* d_mr[kRankVariants] = {};
+  size_t flds = 0, rlds = 0;
+};
 
 struct BatchPlan {
   int count = 0;
@@ -45,10 +65,8 @@ struct BatchPlan {
   std::vector<Work> works;
   std::vector<InstDesc> descs;
   InstDesc* d_desc = nullptr;
-  BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_inst = nullptr;
-  int nms = 0, nmf = 0, nmr = 0;
-  int n_variant[kRankVariants] = {};  // instances per rank kernel variant
-  size_t flds = 0, rlds = 0;
+  BlockRef* d_inst = nullptr;
+  std::vector<Segment> segs;
   int* d_res = nullptr;                 // count x 4
   unsigned long long* d_exp = nullptr;  // count
   int* d_seq = nullptr;                 // sum of L
@@ -59,11 +77,7 @@ struct BatchPlan {
 int grow_batch_arena(jssp_engine* e, size_t bytes) {
   if (bytes <= e->bcap) return JSSP_OK;
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
-  if (e->step_graph) {
-    (void)hipGraphExecDestroy(e->step_graph);
-    e->step_graph = nullptr;
-    e->step_graph_key.clear();
-  }
+  release_step_graphs(e);  // they name buffers of the old arena
   if (e->bbuf) JSSP_HIP(e, hipFree(e->bbuf));
   e->bbuf = nullptr;
   e->bcap = 0;
@@ -85,7 +99,8 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
   std::vector<jssp_params> ps(params, params + count);
   size_t ops_total = 0, seq_total = 0, work_total = 0;
   std::vector<size_t> work_need(count);
-  std::vector<BlockRef> ms, mf, mr, mi;
+  std::vector<BlockRef> mi;
+  std::vector<int> ends;
   for (int i = 0; i < count; ++i) {
     ps[i].phases = 2;  // batches materialise survivors only; beams are identical either way
     Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0);
@@ -97,18 +112,45 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     bp.seq_off.push_back(seq_total);
     seq_total += (size_t)s.L;
     bp.Lmax = std::max(bp.Lmax, s.L);
-    append_maps(s, i, ms, mf, mr);
-    bp.n_variant[rank_variant(s, &ps[i])]++;
     mi.push_back(make_int2(i, 0));
-    bp.flds = std::max(bp.flds, filter_lds_bytes(s, &ps[i]));
-    bp.rlds = std::max(bp.rlds, rank_lds_bytes(s));
+    ends.push_back(s.L);
+  }
+  // Segments end where some instance ends; all maps go in one array, in
+  // segment order (score, filter, then the rank blocks of each variant).
+  std::sort(ends.begin(), ends.end());
+  ends.erase(std::unique(ends.begin(), ends.end()), ends.end());
+  std::vector<BlockRef> maps;
+  std::vector<std::vector<size_t>> offs;  // per segment: score, filter, rank per variant
+  for (size_t j = 0; j < ends.size(); ++j) {
+    Segment sg;
+    sg.first = j ? ends[j - 1] : 0;
+    sg.steps = ends[j] - sg.first;
+    std::vector<BlockRef> ms, mf, mr[kRankVariants];
+    for (int i = 0; i < count; ++i) {
+      const Shape& s = bp.shapes[i];
+      if (s.L <= sg.first) continue;  // finished before this segment
+      append_maps(s, i, ms, mf, mr[rank_variant(s, &ps[i])]);
+      sg.flds = std::max(sg.flds, filter_lds_bytes(s, &ps[i]));
+      sg.rlds = std::max(sg.rlds, rank_lds_bytes(s));
+    }
+    std::vector<size_t> o;
+    auto add = [&](const std::vector<BlockRef>& v) {
+      o.push_back(maps.size());
+      maps.insert(maps.end(), v.begin(), v.end());
+    };
+    add(ms);
+    add(mf);
+    for (int v = 0; v < kRankVariants; ++v) add(mr[v]);
+    sg.nms = (int)ms.size();
+    sg.nmf = (int)mf.size();
+    for (int v = 0; v < kRankVariants; ++v) sg.nmr[v] = (int)mr[v].size();
+    offs.push_back(o);
+    bp.segs.push_back(sg);
   }
   Bump sz;
   sz.take<int2>(ops_total);
   sz.take<InstDesc>(count);
-  sz.take<BlockRef>(ms.size());
-  sz.take<BlockRef>(mf.size());
-  sz.take<BlockRef>(mr.size());
+  sz.take<BlockRef>(maps.size());
   sz.take<BlockRef>(mi.size());
   sz.take<int>((size_t)count * 4);
   sz.take<unsigned long long>(count);
@@ -121,10 +163,14 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
   b.base = e->bbuf;
   int2* d_ops = b.take<int2>(ops_total);
   bp.d_desc = b.take<InstDesc>(count);
-  bp.d_ms = b.take<BlockRef>(ms.size());
-  bp.d_mf = b.take<BlockRef>(mf.size());
-  bp.d_mr = b.take<BlockRef>(mr.size());
+  BlockRef* d_maps = b.take<BlockRef>(maps.size());
   bp.d_inst = b.take<BlockRef>(mi.size());
+  for (size_t j = 0; j < bp.segs.size(); ++j) {
+    Segment& sg = bp.segs[j];
+    sg.d_ms = d_maps + offs[j][0];
+    sg.d_mf = d_maps + offs[j][1];
+    for (int v = 0; v < kRankVariants; ++v) sg.d_mr[v] = d_maps + offs[j][2 + v];
+  }
   bp.d_res = b.take<int>((size_t)count * 4);
   bp.d_exp = b.take<unsigned long long>(count);
   bp.d_seq = b.take<int>(seq_total);
@@ -146,12 +192,9 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     d.shared_bound = shared_bound ? bp.d_bound : nullptr;
     bp.descs.push_back(d);
   }
-  bp.nms = (int)ms.size();
-  bp.nmf = (int)mf.size();
-  bp.nmr = (int)mr.size();
   // One staging pass and one copy per region.
   const size_t hbytes = ops_total * sizeof(int2) + (size_t)count * sizeof(InstDesc) +
-                        (ms.size() + mf.size() + mr.size() + mi.size()) * sizeof(BlockRef) + 64;
+                        (maps.size() + mi.size()) * sizeof(BlockRef) + 64;
   rc = ensure_host_staging(e, hbytes);
   if (rc) return rc;
   char* h = e->hbuf;
@@ -175,66 +218,79 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     hoff = ops_total * sizeof(int2);
   }
   if ((rc = put(bp.d_desc, bp.descs.data(), (size_t)count * sizeof(InstDesc)))) return rc;
-  if ((rc = put(bp.d_ms, ms.data(), ms.size() * sizeof(BlockRef)))) return rc;
-  if ((rc = put(bp.d_mf, mf.data(), mf.size() * sizeof(BlockRef)))) return rc;
-  if ((rc = put(bp.d_mr, mr.data(), mr.size() * sizeof(BlockRef)))) return rc;
+  if ((rc = put(d_maps, maps.data(), maps.size() * sizeof(BlockRef)))) return rc;
   if ((rc = put(bp.d_inst, mi.data(), mi.size() * sizeof(BlockRef)))) return rc;
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
   return JSSP_OK;
 }
 
-int enqueue_step(jssp_engine* e, const BatchPlan& bp) {
-  if (bp.nms > 0) JSSP_HIP(e, launch_step_score(bp.d_desc, bp.d_ms, bp.nms, false, e->stream));
-  JSSP_HIP(e, launch_step_filter(bp.d_desc, bp.d_mf, bp.nmf, bp.flds, e->stream));
-  // Each rank kernel variant skips the other variants' instances.
+int enqueue_step(jssp_engine* e, const BatchPlan& bp, const Segment& sg) {
+  if (sg.nms > 0) JSSP_HIP(e, launch_step_score(bp.d_desc, sg.d_ms, sg.nms, false, e->stream));
+  JSSP_HIP(e, launch_step_filter(bp.d_desc, sg.d_mf, sg.nmf, sg.flds, e->stream));
   for (int v = 0; v < kRankVariants; ++v)
-    if (bp.n_variant[v] > 0) JSSP_HIP(e, launch_step_rank(bp.d_desc, bp.d_mr, bp.nmr, bp.rlds, v, e->stream));
+    if (sg.nmr[v] > 0) JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, e->stream));
   return JSSP_OK;
 }
 
-// kStepsPerGraph steps captured once per batch layout and replayed.
-int enqueue_steps(jssp_engine* e, const BatchPlan& bp, int steps, bool use_graph) {
-  if (!use_graph) {
-    for (int t = 0; t < steps; ++t) {
-      int rc = enqueue_step(e, bp);
+// n steps of segment sg: whole groups of kStepsPerGraph steps replay the
+// segment's captured graph (captured once per launch layout), the rest are
+// launched directly.
+int enqueue_segment(jssp_engine* e, const BatchPlan& bp, const Segment& sg, int n, bool use_graph) {
+  int done = 0;
+  if (use_graph && n >= kStepsPerGraph) {
+    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)bp.d_desc, (uint64_t)sg.nms, (uint64_t)sg.nmf,
+                                 (uint64_t)(uintptr_t)sg.d_ms, (uint64_t)(uintptr_t)sg.d_mf, (uint64_t)sg.flds,
+                                 (uint64_t)sg.rlds};
+    for (int v = 0; v < kRankVariants; ++v) {
+      key.push_back((uint64_t)sg.nmr[v]);
+      key.push_back((uint64_t)(uintptr_t)sg.d_mr[v]);
+    }
+    hipGraphExec_t ge = nullptr;
+    for (const auto& g : e->step_graphs)
+      if (g.first == key) { ge = g.second; break; }
+    if (!ge) {
+      hipGraph_t g = nullptr;
+      JSSP_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int rc = JSSP_OK;
+      for (int t = 0; t < kStepsPerGraph && rc == JSSP_OK; ++t) rc = enqueue_step(e, bp, sg);
+      hipError_t ce = hipStreamEndCapture(e->stream, &g);
       if (rc) return rc;
+      if (ce != hipSuccess) return set_error(e, JSSP_ERR_HIP, "hipStreamEndCapture", hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) return set_error(e, JSSP_ERR_HIP, "hipGraphInstantiate", hipGetErrorString(ie));
+      if (e->step_graphs.size() >= kGraphCacheMax) {
+        // Full cache: let the enqueued launches finish, then drop it.
+        JSSP_HIP(e, hipStreamSynchronize(e->stream));
+        release_step_graphs(e);
+      }
+      e->step_graphs.push_back({key, ge});
     }
-    return JSSP_OK;
+    for (; done + kStepsPerGraph <= n; done += kStepsPerGraph) JSSP_HIP(e, hipGraphLaunch(ge, e->stream));
   }
-  const std::vector<uint64_t> key = {(uint64_t)(uintptr_t)bp.d_desc, (uint64_t)bp.nms, (uint64_t)bp.nmf,
-                                     (uint64_t)bp.nmr, (uint64_t)bp.flds, (uint64_t)bp.rlds,
-                                     (uint64_t)(uintptr_t)bp.d_ms, (uint64_t)(uintptr_t)bp.d_mf,
-                                     (uint64_t)(uintptr_t)bp.d_mr, (uint64_t)bp.n_variant[0],
-                                     (uint64_t)bp.n_variant[1], (uint64_t)bp.n_variant[2],
-                                     (uint64_t)bp.n_variant[3], (uint64_t)bp.n_variant[4]};
-  if (!e->step_graph || e->step_graph_key != key) {
-    if (e->step_graph) {
-      JSSP_HIP(e, hipGraphExecDestroy(e->step_graph));
-      e->step_graph = nullptr;
-    }
-    hipGraph_t g = nullptr;
-    JSSP_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    int rc = JSSP_OK;
-    for (int t = 0; t < kStepsPerGraph && rc == JSSP_OK; ++t) rc = enqueue_step(e, bp);
-    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+  for (; done < n; ++done) {
+    int rc = enqueue_step(e, bp, sg);
     if (rc) return rc;
-    if (ce != hipSuccess) return set_error(e, JSSP_ERR_HIP, "hipStreamEndCapture", hipGetErrorString(ce));
-    hipError_t ie = hipGraphInstantiate(&e->step_graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ie != hipSuccess) {
-      e->step_graph = nullptr;
-      return set_error(e, JSSP_ERR_HIP, "hipGraphInstantiate", hipGetErrorString(ie));
-    }
-    e->step_graph_key = key;
   }
-  // Steps past an instance's last level are no-ops for it.
-  for (int t = 0; t < steps; t += kStepsPerGraph) JSSP_HIP(e, hipGraphLaunch(e->step_graph, e->stream));
+  return JSSP_OK;
+}
+
+// Steps [from, from + n) of the batch (steps past every instance's last
+// level are no-ops and are skipped).
+int enqueue_steps(jssp_engine* e, const BatchPlan& bp, int from, int n, bool use_graph) {
+  const int end = from + n;
+  for (const Segment& sg : bp.segs) {
+    const int lo = std::max(from, sg.first), hi = std::min(end, sg.first + sg.steps);
+    if (lo >= hi) continue;
+    int rc = enqueue_segment(e, bp, sg, hi - lo, use_graph);
+    if (rc) return rc;
+  }
   return JSSP_OK;
 }
 
 int run_plan(jssp_engine* e, const BatchPlan& bp, bool use_graph) {
   JSSP_HIP(e, launch_step_init(bp.d_desc, bp.d_inst, bp.count, e->stream));
-  int rc = enqueue_steps(e, bp, bp.Lmax, use_graph);
+  int rc = enqueue_steps(e, bp, 0, bp.Lmax, use_graph);
   if (rc) return rc;
   JSSP_HIP(e, launch_step_backtrack(bp.d_desc, bp.d_inst, bp.count, e->stream));
   return JSSP_OK;
@@ -322,10 +378,10 @@ int autotune_classes(jssp_engine* e, int count, const jssp_instance* insts, cons
       int rc = plan_batch(e, (int)sub.size(), sub.data(), subp.data(), lanes.data(), false, bp);
       if (rc) return rc;
       JSSP_HIP(e, launch_step_init(bp.d_desc, bp.d_inst, bp.count, e->stream));
-      rc = enqueue_steps(e, bp, 2, false);  // warm up
+      rc = enqueue_steps(e, bp, 0, 2, false);  // warm up
       if (rc) return rc;
       JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
-      rc = enqueue_steps(e, bp, steps, false);
+      rc = enqueue_steps(e, bp, 2, steps, false);
       if (rc) return rc;
       JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
       JSSP_HIP(e, hipStreamSynchronize(e->stream));

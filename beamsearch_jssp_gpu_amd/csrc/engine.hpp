@@ -38,8 +38,8 @@ struct jssp_engine {
   // and the captured step graph of the last batch layout.
   char* bbuf = nullptr;
   size_t bcap = 0;
-  hipGraphExec_t step_graph = nullptr;
-  std::vector<uint64_t> step_graph_key;
+  // Captured step graphs of batch segments, keyed by their launch layout.
+  std::vector<std::pair<std::vector<uint64_t>, hipGraphExec_t>> step_graphs;
   int autotune = 1;       // consult/extend the lanes-per-parent tuning table
   std::string tune_path;  // JSON file of the tuning table ("" = in memory only)
   std::vector<std::pair<uint64_t, int>> tune;  // (n, m, K bucket) -> lanes per parent
