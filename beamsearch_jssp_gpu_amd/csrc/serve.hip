@@ -70,7 +70,9 @@ struct jssp_server {
   uint64_t next_ticket = 1;
   bool maps_dirty = true;
   int nms = 0, nmf = 0, nmr = 0, nact = 0;
-  int n_variant[jssp::kRankVariants] = {};  // resident requests per rank kernel variant
+  // Rank blocks grouped by kernel variant: variant v's blocks are
+  // d_mr[mr_off[v], mr_off[v] + mr_cnt[v]).
+  int mr_off[jssp::kRankVariants] = {}, mr_cnt[jssp::kRankVariants] = {};
   size_t flds = 0, rlds = 0;
   std::vector<InstDesc> hdesc;
   // Pipelined steps: each step's last kernel writes every resident slot's
@@ -203,24 +205,26 @@ int admit(jssp_server* s) {
 
 int rebuild_maps(jssp_server* s) {
   jssp_engine* e = s->e;
-  std::vector<BlockRef> ms, mf, mr, ma;
-  int nv[jssp::kRankVariants] = {};
+  std::vector<BlockRef> ms, mf, mr, ma, mrv[jssp::kRankVariants];
   s->flds = 0;
   s->rlds = 0;
   for (int i = 0; i < s->nslots; ++i) {
     if (!s->slot_ticket[i]) continue;
     const Shape& sh = s->slot_shape[i];
-    jssp::append_maps(sh, i, ms, mf, mr);
-    nv[jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params)]++;
+    jssp::append_maps(sh, i, ms, mf, mrv[jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params)]);
     ma.push_back(make_int2(i, 0));
     const Request& r = s->reqs[s->slot_ticket[i]];
     s->flds = std::max(s->flds, jssp::filter_lds_bytes(sh, &r.params));
     s->rlds = std::max(s->rlds, jssp::rank_lds_bytes(sh));
   }
+  for (int v = 0; v < jssp::kRankVariants; ++v) {
+    s->mr_off[v] = (int)mr.size();
+    s->mr_cnt[v] = (int)mrv[v].size();
+    mr.insert(mr.end(), mrv[v].begin(), mrv[v].end());
+  }
   s->nms = (int)ms.size();
   s->nmf = (int)mf.size();
   s->nmr = (int)mr.size();
-  for (int v = 0; v < jssp::kRankVariants; ++v) s->n_variant[v] = nv[v];
   s->nact = (int)ma.size();
   if (ms.size() > s->cap_map || mf.size() > s->cap_map || mr.size() > s->cap_map)
     return jssp::set_error(e, JSSP_ERR_LIMIT, "serve", "block map capacity exceeded");
@@ -407,8 +411,9 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
       if (s->nms > 0) JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, false, e->stream));
       JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, e->stream));
       for (int v = 0; v < jssp::kRankVariants; ++v)
-        if (s->n_variant[v] > 0)
-          JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr, s->nmr, s->rlds, v, e->stream));
+        if (s->mr_cnt[v] > 0)
+          JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds, v,
+                                             e->stream));
     }
     // This step's status, read by the next serve step (or by this one when
     // it runs no levels).
