@@ -162,6 +162,8 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.rstat = b.take<RankStat>(s.nblk_rank);
   w.sel[0] = b.take<SelState>(1);
   w.sel[1] = b.take<SelState>(1);
+  w.sctr[0] = b.take<uint32_t>((size_t)kSureSubs * kCtrStride);
+  w.sctr[1] = b.take<uint32_t>((size_t)kSureSubs * kCtrStride);
   w.pcount = b.take<int>(s.L + 1);
   w.trail = b.take<int>((size_t)std::max(1, s.L - 1) * s.K);
   w.jump = s.exact ? nullptr : b.take<uint16_t>((size_t)((std::max(1, s.L - 1) + 31) / 32) * s.K);
@@ -217,6 +219,8 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.bstat = w.bstat;
   a.rstat = w.rstat;
   a.sel = w.sel[0];
+  a.sctr = w.sctr[0];
+  a.sctr_next = w.sctr[1];
   a.sel_prev = w.sel[1];
   a.sel_next = w.sel[1];
   a.sel_log = nullptr;
@@ -250,6 +254,7 @@ InstDesc make_desc(const Shape& s, const Work& w, const jssp_params* p, const js
   d.beam[0] = w.beam[0]; d.beam[1] = w.beam[1];
   d.hist[0] = w.hist[0]; d.hist[1] = w.hist[1];
   d.sel[0] = w.sel[0]; d.sel[1] = w.sel[1];
+  d.sctr[0] = w.sctr[0]; d.sctr[1] = w.sctr[1];
   d.crec[0] = w.crec[0]; d.crec[1] = w.crec[1];
   d.nblk_score = s.nblk_score;
   d.nblk_filter = s.nblk_filter;

@@ -227,6 +227,10 @@ __global__ void k_init(LevelArgs a) {
     a.hist[i] = 0;
     a.hist_next[i] = 0;
   }
+  for (int i = threadIdx.x; i < kSureSubs; i += blockDim.x) {
+    a.sctr[i * kCtrStride] = 0u;
+    a.sctr_next[i * kCtrStride] = 0u;
+  }
   if (threadIdx.x == 0) {
     SelState z = {};
     *a.sel = z;
@@ -1078,6 +1082,9 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
 
   GL u64* cand = gp(a.cand);
   GL u64* tiles_keys = gp(a.ref_buf);
+  const bool subs = sure_subs(single, a.K, a.B);
+  const uint32_t sub = (uint32_t)local % kSureSubs;
+  uint32_t* sctr = a.sctr + sub * kCtrStride;
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
@@ -1102,7 +1109,8 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
         s_off[0][w] = ts; s_off[1][w] = tt;
         ts += cs; tt += ct;
       }
-      s_base[0] = ts ? atomicAdd(&sel->sure_ctr, ts) : 0u;
+      if (subs) s_base[0] = ts ? sub * (uint32_t)a.K + atomicAdd(sctr, ts) : 0u;
+      else s_base[0] = ts ? atomicAdd(&sel->sure_ctr, ts) : 0u;
       if (single) gp(a.tcnt)[base / blockDim.x] = tt;
       else s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
     }
@@ -1265,6 +1273,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     sn->sure_ctr = 0;
     sn->tie_ctr = 0;
     sn->done_ctr = 0;
+    for (int r = 0; r < kSureSubs; ++r) gp(a.sctr_next)[r * kCtrStride] = 0u;
     if (a.sel_log) a.sel_log[a.level] = *a.sel;
   }
   // The selection state, this thread's first tie count and the survivor keys
@@ -1277,15 +1286,43 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   GL const u64* cand = gp(a.cand);
   GL const uint32_t* tcnt = gp(a.tcnt);
   GL const SelState* sel = gp(a.sel);
-  const uint32_t kp_raw = sel->Kp, sf_raw = sel->sorted_from;
+  const uint32_t kp_raw = sel->Kp, sf_raw = sel->sorted_from, single_raw = sel->single;
   const uint32_t N = (uint32_t)gp(a.pcount)[a.level] * (uint32_t)a.B;
   const uint32_t tc0 = t0 < tiles_max ? tcnt[t0] : 0u;
-  if (in_lds) {
-#pragma unroll 4
-    for (int i = tid; i < a.K; i += blockDim.x) s_c[i] = cand[i];
-  }
+  // The keys below the threshold bin: one run per counter region when the
+  // filter spread them (their counts add up to sf), else cand[0, sf).
+  uint32_t sc[kSureSubs];
+  GL const uint32_t* sctr = gp(a.sctr);
+#pragma unroll
+  for (int r = 0; r < kSureSubs; ++r) sc[r] = sctr[r * kCtrStride];
   const int Kp = (int)kp_raw;
   const int sf = min((int)sf_raw, Kp);
+  if (in_lds) {
+    if (sure_subs(single_raw != 0, a.K, a.B)) {
+      // Entry i of the gathered run lies in region r (the last r whose runs
+      // before it hold <= i keys), at cand[r*K + i - (keys of regions < r)]:
+      // i plus (K - c[q-1]) for every region boundary q at or below i.
+      uint32_t bound[kSureSubs], skip[kSureSubs];
+      uint32_t run = 0;
+#pragma unroll
+      for (int q = 1; q < kSureSubs; ++q) {
+        const uint32_t c = min(sc[q - 1], (uint32_t)a.K);
+        run += c;
+        bound[q] = run;
+        skip[q] = (uint32_t)a.K - c;
+      }
+#pragma unroll 4
+      for (int i = tid; i < sf; i += blockDim.x) {
+        uint32_t src = (uint32_t)i;
+#pragma unroll
+        for (int q = 1; q < kSureSubs; ++q) src += (uint32_t)i >= bound[q] ? skip[q] : 0u;
+        s_c[i] = cand[src];
+      }
+    } else {
+#pragma unroll 4
+      for (int i = tid; i < a.K; i += blockDim.x) s_c[i] = cand[i];
+    }
+  }
   const bool ties = sf < Kp;
   uint32_t* s_tp = reinterpret_cast<uint32_t*>(s_c + (in_lds ? a.K : 0));
   const int tiles = (int)((N + kFilterThreads - 1) / kFilterThreads);
@@ -1581,6 +1618,8 @@ __device__ __forceinline__ LevelArgs resolve(const InstDesc& d, int level) {
   a.hist = odd ? d.hist[1] : d.hist[0];
   a.hist_next = odd ? d.hist[0] : d.hist[1];
   a.sel = odd ? d.sel[1] : d.sel[0];
+  a.sctr = odd ? d.sctr[1] : d.sctr[0];
+  a.sctr_next = odd ? d.sctr[0] : d.sctr[1];
   a.sel_prev = odd ? d.sel[0] : d.sel[1];
   a.sel_next = odd ? d.sel[0] : d.sel[1];
   a.crec = odd ? d.crec[1] : d.crec[0];
@@ -1607,6 +1646,10 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
     d.hist[1][i] = 0;
   }
   for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
+  for (int i = threadIdx.x; i < kSureSubs; i += blockDim.x) {
+    d.sctr[0][i * kCtrStride] = 0u;
+    d.sctr[1][i * kCtrStride] = 0u;
+  }
   if (threadIdx.x == 0) {
     // Multi-seed passes prune with the shared bound as it stands when the
     // pass starts, so every beam of a pass sees the same bound.

@@ -30,6 +30,17 @@ constexpr int kRankLdsMax = 16384;
 // Dynamic LDS the fused rank kernel may use (its static LDS is ~9 KB of 160 KB);
 // solves that would need more use the separate score kernel.
 constexpr size_t kRankFusedLdsMax = 144 * 1024;
+// The filter reserves the slots of the keys below the threshold bin with
+// atomics on kSureSubs counters (filter block b uses counter b % kSureSubs,
+// its keys going to cand[r*K, (r+1)*K)), kCtrStride words apart, instead of
+// one counter that every block hits in turn.  Used when the threshold bin is
+// single-valued, K <= kRankLdsMax and B >= kSureSubs; the rank kernel gathers
+// the regions in order.
+constexpr int kSureSubs = 8;
+constexpr int kCtrStride = 64;
+__host__ __device__ inline bool sure_subs(bool single_bin, int K, int B) {
+  return single_bin && K <= kRankLdsMax && B >= kSureSubs;
+}
 // LDS bitmap over child indices for the threshold cut: up to 2^20 children.
 constexpr uint32_t kBitmapWordsMax = 32768;
 
@@ -76,6 +87,8 @@ struct LevelArgs {
   uint32_t* hist_next;       // kHistCopies x kNB, zeroed for the next level
   BlockStat* bstat;          // nblk_score partials
   RankStat* rstat;           // nblk_rank partials of the previous level's survivors
+  uint32_t* sctr;            // this level's sure-key counters (kSureSubs, kCtrStride words apart)
+  uint32_t* sctr_next;       // the next level's (zeroed by the rank kernel)
   SelState* sel;             // this level
   const SelState* sel_prev;  // previous level
   SelState* sel_next;        // next level (counters zeroed by the rank kernel)
@@ -115,6 +128,7 @@ struct InstDesc {
   uint32_t* hist[2];
   SelState* sel[2];
   unsigned long long* crec[2];
+  uint32_t* sctr[2];
   int nblk_score, nblk_filter, nblk_rank;
   // Level each step kernel runs next.  Each counter is advanced by block 0
   // of the kernel that follows its reader in the step (score -> filter ->

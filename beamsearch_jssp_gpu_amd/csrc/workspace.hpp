@@ -25,6 +25,7 @@ struct Work {
   BlockStat* bstat;
   RankStat* rstat;
   SelState* sel[2];
+  uint32_t* sctr[2];
   int* pcount;
   int* trail;
   uint16_t* jump;
