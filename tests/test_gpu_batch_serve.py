@@ -136,3 +136,24 @@ def test_serving_budget_and_cancel(engine):
     want = engine.solve(_b(inst), 32, mode=bs.MODE_ICH_DET, seed=3)
     _same(srv.poll(t3), want)
     srv.close()
+
+
+def test_serving_steps_are_pipelined(engine):
+    """A serve step returns with its levels still running and collects what
+    the previous step finished: a request whose last level runs in one step
+    is reported by the next call; a step of no levels waits for everything
+    enqueued.  Results equal the plain solve either way."""
+    inst = splitmix_instance(77, 10, 5, 1, 99)  # 50 levels
+    p = dict(beam_width=64, mode=bs.MODE_PMS_GPU, phases=2, best_known=bs.NO_MAKESPAN, seed=3)
+    alone = engine.solve(_b(inst), **p)
+    srv = bs.Server(engine, hbm_budget_bytes=64 << 20, max_slots=4)
+    t = srv.submit(_b(inst), 64, bs.MODE_PMS_GPU, seed=3)
+    assert srv.step(64) == 1  # every level enqueued; nothing collected yet
+    assert srv.poll(t) is None
+    assert srv.step(64) == 0  # collects the previous step
+    _same(srv.poll(t), alone)
+    t2 = srv.submit(_b(inst), 64, bs.MODE_PMS_GPU, seed=3)
+    srv.step(64)
+    assert srv.step(0) == 0  # no levels: waits and collects
+    _same(srv.poll(t2), alone)
+    srv.close()
