@@ -262,8 +262,13 @@ struct JobView {
 };
 
 // Where a score pass writes, and the per-level constants it needs.
+// keys: the FP32 score bits of each child slot (the child index is the
+// slot's position: the filter forms (bits << 32 | x) as it reads them), or
+// kNoScore for an invalid or pruned slot.
+constexpr uint32_t kNoScore = 0xFFFFFFFFu;  // no finite positive float has these bits
+
 struct ScoreDst {
-  GL u64* keys;
+  GL uint32_t* keys;
   GL uint16_t* kdig;
   GL u64* crec;
   uint32_t lseed;
@@ -271,7 +276,7 @@ struct ScoreDst {
 };
 __device__ __forceinline__ ScoreDst score_dst(const LevelArgs& a) {
   ScoreDst d;
-  d.keys = gp(a.keys);
+  d.keys = reinterpret_cast<GL uint32_t*>(gp(a.keys));
   d.kdig = gp(a.kdig);
   d.crec = gp(a.crec);
   d.lseed = a.lseed;
@@ -461,7 +466,7 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
           const float base = gpu_score_base(v.cm[h], sl.ns);
           const float sc = gpu_score_add_noise(base, noise_hash(a.lseed, (uint32_t)p, (uint32_t)slot[h]));
           const uint32_t bits = __float_as_uint(sc);
-          a.keys[x] = ((u64)bits << 32) | x;
+          a.keys[x] = bits;
           if (a.exact_digit) {
             // Monotone in (score, child) when a lower cmax always scores
             // lower (checked per instance on the host): cmax >= lo, and for
@@ -478,14 +483,14 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
           acc.kmin = min(acc.kmin, v.cm[h]);
         }
       } else if (!sl.last) {
-        a.keys[x] = ~0ull;
+        a.keys[x] = kNoScore;
       }
     }
     lds_count(s_h, d, counted, lane);
   }
   if (!sl.last && pact) {
     for (int s = gl; s < B; s += G)
-      if (s >= nvalid) a.keys[(size_t)p * B + s] = ~0ull;
+      if (s >= nvalid) a.keys[(size_t)p * B + s] = kNoScore;
   }
 }
 
@@ -1011,11 +1016,11 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   // score-block partial per thread (nblk_score <= 512), the level's parents,
   // and this block's first tile of keys and digits (read within the buffers'
   // K*B capacity; slots past this level's count are ignored below).
-  GL const u64* keys = gp(a.keys);
+  GL const uint32_t* keys = reinterpret_cast<GL const uint32_t*>(gp(a.keys));
   GL const uint16_t* kdig = gp(a.kdig);
   const uint32_t cap = (uint32_t)a.K * (uint32_t)a.B;
   const uint32_t xf = (uint32_t)local * blockDim.x + tid;
-  const u64 key_first = xf < cap ? keys[xf] : ~0ull;
+  const uint32_t key_first = xf < cap ? keys[xf] : kNoScore;
   const uint32_t dg_first = xf < cap ? (uint32_t)kdig[xf] : (uint32_t)kNB;
   uint32_t h0 = 0, h1 = 0;
 #pragma unroll
@@ -1091,7 +1096,8 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     const bool first = base == (uint32_t)local * blockDim.x;
     // Both loads at once (the first tile's were issued in the prologue): the
     // digit of an invalid slot is never read.
-    const u64 key = inb ? (first ? key_first : keys[x]) : ~0ull;
+    const uint32_t sb = inb ? (first ? key_first : keys[x]) : kNoScore;
+    const u64 key = sb != kNoScore ? (((u64)sb << 32) | x) : ~0ull;
     const uint32_t dg = inb ? (first ? dg_first : (uint32_t)kdig[x]) : (uint32_t)kNB;
     const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;

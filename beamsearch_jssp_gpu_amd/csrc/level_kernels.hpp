@@ -76,7 +76,8 @@ struct LevelArgs {
   const int* beam_cur;       // P x S
   int* beam_next;            // K x S
   int* cbuf;                 // single-phase children, (K*B) x S
-  unsigned long long* keys;  // K*B (also the refinement scratch)
+  unsigned long long* keys;  // K*B 8-byte slots: child x's FP32 score bits as u32 at word x (the filter
+                             // forms the (score, x) key), then the refinement scratch (u64)
   unsigned long long* crec;  // K*B: kept child's record, finish | machine << 32 | job << 40 | next
                              // machine << 48 (two buffers by level parity)
   unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
