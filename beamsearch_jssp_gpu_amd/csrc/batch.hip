@@ -103,7 +103,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
   std::vector<int> ends;
   for (int i = 0; i < count; ++i) {
     ps[i].phases = 2;  // batches materialise survivors only; beams are identical either way
-    Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0);
+    Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0, kRankWavesThroughput);
     bp.shapes.push_back(s);
     Work w;
     work_need[i] = carve(s, nullptr, w);
@@ -228,7 +228,8 @@ int enqueue_step(jssp_engine* e, const BatchPlan& bp, const Segment& sg) {
   if (sg.nms > 0) JSSP_HIP(e, launch_step_score(bp.d_desc, sg.d_ms, sg.nms, false, e->stream));
   JSSP_HIP(e, launch_step_filter(bp.d_desc, sg.d_mf, sg.nmf, sg.flds, e->stream));
   for (int v = 0; v < kRankVariants; ++v)
-    if (sg.nmr[v] > 0) JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, e->stream));
+    if (sg.nmr[v] > 0)
+      JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, kRankWavesThroughput, e->stream));
   return JSSP_OK;
 }
 

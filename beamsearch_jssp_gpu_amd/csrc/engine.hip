@@ -105,21 +105,21 @@ int default_lanes(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 // Dynamic LDS of the rank kernel: survivor keys (K <= kRankLdsMax), the tile
 // prefix of tie counts (padded to 16 bytes) and, fused, one snapshot view per
 // wave.
-static size_t rank_lds_for(int K, int n, int m, bool fused) {
+static size_t rank_lds_for(int K, int n, int m, bool fused, int waves) {
   const size_t tiles = ((size_t)K * n + kFilterThreads - 1) / kFilterThreads;
   size_t b = (K <= kRankLdsMax ? (size_t)K * 8 : 0) + 4 * ((tiles + 1 + 3) & ~(size_t)3);
-  if (fused) b += (size_t)kRankWaves * (2 * n + m + 1) * 4;
+  if (fused) b += (size_t)waves * (2 * n + m + 1) * 4;
   return b;
 }
 
 bool fused_eligible(const jssp_instance* in, const jssp_params* p) {
   const bool exact = (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT);
   return !exact && p->phases == 2 && !(p->flags & (JSSP_FLAG_GENERIC_DIGIT | JSSP_FLAG_UNFUSED)) &&
-         rank_lds_for(p->beam_width, in->n_jobs, in->n_machines, true) <= kRankFusedLdsMax &&
+         rank_lds_for(p->beam_width, in->n_jobs, in->n_machines, true, kRankWaves) <= kRankFusedLdsMax &&
          exact_digit_applies(in);
 }
 
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes) {
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves) {
   Shape s;
   s.n = in->n_jobs; s.m = in->n_machines;
   s.S = 2 * s.n + s.m + 1;
@@ -137,7 +137,8 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes) {
   const int parents_per_block = kScoreWaves * (64 / s.G);
   s.nblk_score = (int)std::max(1, std::min((s.K + parents_per_block - 1) / parents_per_block, 512));
   s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads, 512ll));
-  s.nblk_rank = (int)std::max(1, std::min((s.K + kRankWaves - 1) / kRankWaves, 256));
+  s.rank_waves = rank_waves == kRankWavesThroughput ? kRankWavesThroughput : kRankWaves;
+  s.nblk_rank = (int)std::max(1, std::min((s.K + s.rank_waves - 1) / s.rank_waves, 16 * 256 / s.rank_waves));
   return s;
 }
 
@@ -282,7 +283,7 @@ size_t filter_lds_bytes(const Shape& s, const jssp_params* p) {
   return 4 * std::min<size_t>(((size_t)s.K * s.B + 31) / 32, kBitmapWordsMax);
 }
 
-size_t rank_lds_bytes(const Shape& s) { return rank_lds_for(s.K, s.n, s.m, s.fused); }
+size_t rank_lds_bytes(const Shape& s) { return rank_lds_for(s.K, s.n, s.m, s.fused, s.rank_waves); }
 
 int rank_variant(const Shape& s, const jssp_params* p) {
   if (!s.fused) return kRankUnfused;
@@ -390,7 +391,7 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
                                     prof.pair(kKindScore, &ep)));
     JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, e->stream,
                                    prof.pair(kKindFilter, &ep, l < s.L - 1)));
-    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, rank_variant(s, &q), e->stream,
+    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, rank_variant(s, &q), s.rank_waves, e->stream,
                                  prof.pair(kKindRank, &ep, l < s.L - 1)));
   }
   JSSP_HIP(e, launch_step_jump(d_desc, s.K, s.L, e->stream));

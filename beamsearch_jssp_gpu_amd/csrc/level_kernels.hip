@@ -1262,7 +1262,7 @@ __device__ __forceinline__ uint32_t count_below(P C, int n, u64 key, int lane) {
 // scores its children for the next level.
 // SMALL (fused only): S <= 128 and n <= 64, so a snapshot is two words per
 // lane and a job one slot per lane.
-template <bool FUSED, int MODE, bool SMALL = false>
+template <bool FUSED, int MODE, bool SMALL, int W>
 __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nblk) {
   constexpr int NWM = SMALL ? 2 : kMaxWPL;
   constexpr int NHM = SMALL ? 1 : 2;
@@ -1371,7 +1371,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   ScoreAcc acc;
   int w_min = INT_MAX;
   uint32_t w_max = 0, w_minh = 0xFFFFFFFFu;
-  const int nwaves = kRankWaves;
+  const int nwaves = W;
   const bool single = !FUSED && a.cbuf != nullptr;
   const int order = MODE == 1 ? kOrderErase : (MODE == 2 ? kOrderSwap : ((a.mode == 1) ? kOrderErase : kOrderSwap));
   for (int u = local * nwaves + wave; u < Kp; u += nblk * nwaves) {
@@ -1748,7 +1748,8 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, con
 
 // Rank + materialise; instances of the other form return at once (a batch
 // mixing both forms launches both kernels over the same map).
-__global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, const BlockRef* map) {
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_rank_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   if (d.fused) return;
@@ -1758,12 +1759,12 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_b(InstDesc* desc, cons
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<false, 0>(a, nullptr, r.y, nblk);
+  rank_body<false, 0, false, W>(a, nullptr, r.y, nblk);
 }
 
 // Rank + materialise + score of the next level (fused form), per mode.
-template <int MODE, bool SMALL>
-__global__ __launch_bounds__(64 * kRankWaves) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
+template <int MODE, bool SMALL, int W>
+__global__ __launch_bounds__(64 * W) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   // one round of loads for the variant test (no short-circuit)
@@ -1776,7 +1777,7 @@ __global__ __launch_bounds__(64 * kRankWaves) void k_rank_fused_b(InstDesc* desc
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<true, MODE, SMALL>(a, crec_next, r.y, nblk);
+  rank_body<true, MODE, SMALL, W>(a, crec_next, r.y, nblk);
 }
 
 // Jump table of a single solve (base.jump): thread (g, q) follows survivor
@@ -1854,17 +1855,24 @@ hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, siz
   return hipGetLastError();
 }
 
-hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
-                            hipStream_t s, const EventPair* ev) {
-  void (*k)(InstDesc*, const BlockRef*) = k_rank_b;
+template <int W>
+static void (*rank_kernel(int variant))(InstDesc*, const BlockRef*) {
   switch (variant) {
-    case kRankFusedIch: k = k_rank_fused_b<1, false>; break;
-    case kRankFusedPms: k = k_rank_fused_b<2, false>; break;
-    case kRankFusedIchSmall: k = k_rank_fused_b<1, true>; break;
-    case kRankFusedPmsSmall: k = k_rank_fused_b<2, true>; break;
-    default: break;
+    case kRankFusedIch: return k_rank_fused_b<1, false, W>;
+    case kRankFusedPms: return k_rank_fused_b<2, false, W>;
+    case kRankFusedIchSmall: return k_rank_fused_b<1, true, W>;
+    case kRankFusedPmsSmall: return k_rank_fused_b<2, true, W>;
+    default: return k_rank_b<W>;
   }
-  launch(k, dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
+}
+
+hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
+                            int waves, hipStream_t s, const EventPair* ev) {
+  if (waves == kRankWavesThroughput)
+    launch(rank_kernel<kRankWavesThroughput>(variant), dim3(nblk), dim3(64 * kRankWavesThroughput), lds_bytes, s,
+           ev, desc, map);
+  else
+    launch(rank_kernel<kRankWaves>(variant), dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
   return hipGetLastError();
 }
 

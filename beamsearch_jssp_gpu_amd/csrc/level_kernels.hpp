@@ -24,6 +24,10 @@ constexpr int kScoreWaves = 16;
 // Threads per filter block (one tile of keys per pass) and waves per rank block.
 constexpr int kFilterThreads = 1024;
 constexpr int kRankWaves = 16;
+// Rank blocks of batches and the server: half the waves, so two blocks share
+// a CU and one block's memory waits overlap another's work (single solves
+// keep one 16-wave block per CU: fewer blocks stage the survivor keys).
+constexpr int kRankWavesThroughput = 8;
 // Survivors ranked in LDS (8 bytes each) up to this beam width; wider beams
 // rank from global memory.
 constexpr int kRankLdsMax = 16384;
@@ -172,8 +176,9 @@ enum RankVariant : int {
   kRankFusedPmsSmall = 4,
   kRankVariants = 5
 };
+// waves: kRankWaves or kRankWavesThroughput (the Shape's rank_waves).
 hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
-                            hipStream_t s, const EventPair* ev = nullptr);
+                            int waves, hipStream_t s, const EventPair* ev = nullptr);
 // status (optional, device-visible host memory): per instance of the map,
 // whether it is finished after this launch.
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s,

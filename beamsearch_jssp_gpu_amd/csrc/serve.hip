@@ -131,7 +131,7 @@ size_t footprint(jssp_server* s, const Request& r, Shape& shape) {
   jssp_params q = r.params;
   q.phases = 2;
   const int lanes = jssp_engine_tuned_lanes(s->e, r.inst.n_jobs, r.inst.n_machines, q.beam_width);
-  shape = jssp::make_shape(&r.inst, &q, lanes);
+  shape = jssp::make_shape(&r.inst, &q, lanes, jssp::kRankWavesThroughput);
   Work w;
   return jssp::carve(shape, nullptr, w);
 }
@@ -413,7 +413,7 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
       for (int v = 0; v < jssp::kRankVariants; ++v)
         if (s->mr_cnt[v] > 0)
           JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds, v,
-                                             e->stream));
+                                             jssp::kRankWavesThroughput, e->stream));
     }
     // This step's status, read by the next serve step (or by this one when
     // it runs no levels).

@@ -47,6 +47,7 @@ struct Shape {
   bool single, exact;
   int G;                       // lanes per parent in the score kernel
   bool fused;                  // the rank kernel scores the next level (no score kernel)
+  int rank_waves;              // waves per rank block: kRankWaves, or kRankWavesThroughput
   int nblk_score, nblk_filter, nblk_rank;
 };
 
@@ -60,7 +61,9 @@ bool fused_eligible(const jssp_instance* in, const jssp_params* p);
 // lanes: 0 = automatic (fused when eligible, else default_lanes), kLanesFused,
 // or 16 / 32 / 64 lanes per parent in a separate score kernel (single-phase
 // always uses a 64-lane score kernel).
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes);
+// rank_waves: kRankWaves (single solves) or kRankWavesThroughput (batches,
+// the server).
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves = kRankWaves);
 // Bytes of the workspace (base == nullptr) or carve it at base.
 size_t carve(const Shape& s, char* base, Work& w);
 LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p);
