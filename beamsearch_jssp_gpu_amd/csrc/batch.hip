@@ -226,7 +226,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
 
 int enqueue_step(jssp_engine* e, const BatchPlan& bp, const Segment& sg) {
   if (sg.nms > 0) JSSP_HIP(e, launch_step_score(bp.d_desc, sg.d_ms, sg.nms, false, e->stream));
-  JSSP_HIP(e, launch_step_filter(bp.d_desc, sg.d_mf, sg.nmf, sg.flds, e->stream));
+  JSSP_HIP(e, launch_step_filter(bp.d_desc, sg.d_mf, sg.nmf, sg.flds, true, e->stream));
   for (int v = 0; v < kRankVariants; ++v)
     if (sg.nmr[v] > 0)
       JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, kRankWavesThroughput, e->stream));

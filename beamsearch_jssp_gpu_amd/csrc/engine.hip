@@ -389,7 +389,7 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
     if (!s.fused)
       JSSP_HIP(e, launch_step_score(d_desc, nullptr, s.nblk_score, s.single, e->stream,
                                     prof.pair(kKindScore, &ep)));
-    JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, e->stream,
+    JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, false, e->stream,
                                    prof.pair(kKindFilter, &ep, l < s.L - 1)));
     JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, rank_variant(s, &q), s.rank_waves, e->stream,
                                  prof.pair(kKindRank, &ep, l < s.L - 1)));

@@ -1735,7 +1735,7 @@ __global__ __launch_bounds__(1024) void k_expand_b(InstDesc* desc, const BlockRe
   score_body<true>(a, r.y, nblk);
 }
 
-__global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
+__device__ __forceinline__ void filter_step(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   const InstDesc d = fetch_desc(desc + r.x);
   const int level = d.lvl_filter;
@@ -1744,6 +1744,16 @@ __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, con
   if (level >= d.base.L - 1) return;  // the last level only records the best child
   const LevelArgs a = resolve(d, level);
   filter_body(a, r.y, d.nblk_filter);
+}
+
+__global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
+  filter_step(desc, map);
+}
+
+// Batches and the server: registers held to 64 per lane (a few spills), so
+// two blocks share a CU and one block's memory waits overlap another's work.
+__global__ __launch_bounds__(kFilterThreads, 8) void k_filter_tp_b(InstDesc* desc, const BlockRef* map) {
+  filter_step(desc, map);
 }
 
 // Rank + materialise; instances of the other form return at once (a batch
@@ -1849,9 +1859,9 @@ hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool
   return hipGetLastError();
 }
 
-hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
-                              const EventPair* ev) {
-  launch(k_filter_b, dim3(nblk), dim3(kFilterThreads), lds_bytes, s, ev, desc, map);
+hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, bool throughput,
+                              hipStream_t s, const EventPair* ev) {
+  launch(throughput ? k_filter_tp_b : k_filter_b, dim3(nblk), dim3(kFilterThreads), lds_bytes, s, ev, desc, map);
   return hipGetLastError();
 }
 

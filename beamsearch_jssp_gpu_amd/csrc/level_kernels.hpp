@@ -162,8 +162,9 @@ struct EventPair {
 // this launch (null: plain launch).
 hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool single, hipStream_t s,
                              const EventPair* ev = nullptr);
-hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, hipStream_t s,
-                              const EventPair* ev = nullptr);
+// throughput: the batch / server form of the kernel (two blocks per CU).
+hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, bool throughput,
+                              hipStream_t s, const EventPair* ev = nullptr);
 // Rank kernel variants: the separate-score form, and the fused form per mode
 // (ICH_DET, PMS_GPU) and size (small: S <= 128 words and n <= 64 jobs).  Each
 // returns at once for instances of another variant, so a batch mixing them

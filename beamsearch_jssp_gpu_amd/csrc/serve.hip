@@ -409,7 +409,7 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
   if (s->nact > 0) {
     for (int t = 0; t < levels; ++t) {
       if (s->nms > 0) JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, false, e->stream));
-      JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, e->stream));
+      JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, true, e->stream));
       for (int v = 0; v < jssp::kRankVariants; ++v)
         if (s->mr_cnt[v] > 0)
           JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds, v,
