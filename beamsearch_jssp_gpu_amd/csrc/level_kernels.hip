@@ -1759,7 +1759,7 @@ __global__ __launch_bounds__(kFilterThreads, 8) void k_filter_tp_b(InstDesc* des
 // Rank + materialise; instances of the other form return at once (a batch
 // mixing both forms launches both kernels over the same map).
 template <int W>
-__global__ __launch_bounds__(64 * W) void k_rank_b(InstDesc* desc, const BlockRef* map) {
+__global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_rank_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   if (d.fused) return;
@@ -1774,7 +1774,7 @@ __global__ __launch_bounds__(64 * W) void k_rank_b(InstDesc* desc, const BlockRe
 
 // Rank + materialise + score of the next level (fused form), per mode.
 template <int MODE, bool SMALL, int W>
-__global__ __launch_bounds__(64 * W) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
+__global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   // one round of loads for the variant test (no short-circuit)
