@@ -1773,8 +1773,11 @@ __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_r
 }
 
 // Rank + materialise + score of the next level (fused form), per mode.
+// Throughput blocks: four per CU for the small variant (64 registers), three
+// for the large one (its spills at 64 cost more than the fourth block gains).
 template <int MODE, bool SMALL, int W>
-__global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_rank_fused_b(InstDesc* desc, const BlockRef* map) {
+__global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SMALL ? 8 : 6) : 4) void k_rank_fused_b(
+    InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   // one round of loads for the variant test (no short-circuit)
