@@ -49,6 +49,24 @@ int ensure_device_arena(jssp_engine* e, size_t bytes) {
   return JSSP_OK;
 }
 
+int ensure_exact_host(jssp_engine* e, size_t bytes) {
+  if (bytes <= e->xcap) return JSSP_OK;
+  if (e->xbuf) {
+    JSSP_HIP(e, hipStreamSynchronize(e->stream));
+    JSSP_HIP(e, hipHostFree(e->xbuf));
+    e->xbuf = nullptr;
+    e->xcap = 0;
+  }
+  const size_t cap = bytes + bytes / 4 + 4096;
+  if (hipHostMalloc(&e->xbuf, cap, hipHostMallocCoherent) != hipSuccess) {
+    e->xbuf = nullptr;
+    (void)hipGetLastError();
+    return set_error(e, JSSP_ERR_NOMEM, "hipHostMalloc", "exact-mode host arrays of " + std::to_string(cap) + " bytes");
+  }
+  e->xcap = cap;
+  return JSSP_OK;
+}
+
 int ensure_host_staging(jssp_engine* e, size_t bytes) {
   if (bytes <= e->hcap) return JSSP_OK;
   if (e->hbuf) {
@@ -477,14 +495,25 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, R
   carve(s, e->dbuf, w);
   rc = upload_ops(e, in, w.ops);
   if (rc) return rc;
+  // The level's children (count, cmax, job, machine per slot) and the heap
+  // order live in coherent pinned host memory that the exact kernels write
+  // and read directly: no copy command per level, one wait for the expand.
   const size_t N = (size_t)s.K * s.B;
-  rc = ensure_host_staging(e, N * (sizeof(int) + 2) + (size_t)s.K * 2 * sizeof(int) + 1024);
+  rc = ensure_exact_host(e, N * (sizeof(int) + 2) + (size_t)s.K * 2 * sizeof(int) + 1024);
   if (rc) return rc;
-  int* h_cmax = reinterpret_cast<int*>(e->hbuf);
+  int* h_cmax = reinterpret_cast<int*>(e->xbuf);
   uint8_t* h_job = reinterpret_cast<uint8_t*>(h_cmax + N);
   uint8_t* h_mc = h_job + N;
   int* h_r = reinterpret_cast<int*>(((uintptr_t)(h_mc + N) + 15) & ~(uintptr_t)15);
   int* h_order = h_r + s.K;
+  char* xdev = nullptr;
+  JSSP_HIP(e, hipHostGetDevicePointer(reinterpret_cast<void**>(&xdev), e->xbuf, 0));
+  auto dev = [&](const void* hp) { return xdev + (reinterpret_cast<const char*>(hp) - e->xbuf); };
+  w.ex_cmax = reinterpret_cast<int*>(dev(h_cmax));
+  w.ex_job = reinterpret_cast<uint8_t*>(dev(h_job));
+  w.ex_mc = reinterpret_cast<uint8_t*>(dev(h_mc));
+  w.ex_rcount = reinterpret_cast<int*>(dev(h_r));
+  w.ex_order = reinterpret_cast<int*>(dev(h_order));
 
   LevelArgs a = base_args(s, w, p);
   JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
@@ -498,10 +527,6 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, R
   for (int l = 0; l < s.L && P > 0; ++l) {
     set_exact_level(a, w, l);
     JSSP_HIP(e, launch_exact_expand(a, P, e->stream));
-    JSSP_HIP(e, hipMemcpyAsync(h_r, w.ex_rcount, (size_t)P * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    JSSP_HIP(e, hipMemcpyAsync(h_cmax, w.ex_cmax, (size_t)P * s.B * sizeof(int), hipMemcpyDeviceToHost, e->stream));
-    JSSP_HIP(e, hipMemcpyAsync(h_job, w.ex_job, (size_t)P * s.B, hipMemcpyDeviceToHost, e->stream));
-    if (!ich) JSSP_HIP(e, hipMemcpyAsync(h_mc, w.ex_mc, (size_t)P * s.B, hipMemcpyDeviceToHost, e->stream));
     JSSP_HIP(e, hipStreamSynchronize(e->stream));
     ++levels;
     heap.clear();
@@ -547,10 +572,7 @@ int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, R
     }
     const int count = (int)heap.size();
     for (int i = 0; i < count; ++i) h_order[i] = heap[i].x;
-    if (count > 0) {
-      JSSP_HIP(e, hipMemcpyAsync(w.ex_order, h_order, (size_t)count * sizeof(int), hipMemcpyHostToDevice, e->stream));
-      JSSP_HIP(e, launch_exact_materialize(a, count, e->stream));
-    }
+    if (count > 0) JSSP_HIP(e, launch_exact_materialize(a, count, e->stream));
     P = count;
   }
   const bool found = best < JSSP_NO_MAKESPAN && best_p >= 0;
@@ -630,6 +652,7 @@ void jssp_engine_destroy(jssp_engine* e) {
   if (e->ev_stop) (void)hipEventDestroy(e->ev_stop);
   if (e->dbuf) (void)hipFree(e->dbuf);
   if (e->hbuf) (void)hipHostFree(e->hbuf);
+  if (e->xbuf) (void)hipHostFree(e->xbuf);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
 }

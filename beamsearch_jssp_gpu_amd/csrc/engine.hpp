@@ -29,6 +29,8 @@ struct jssp_engine {
   size_t dcap = 0;
   char* hbuf = nullptr;   // pinned host staging (grows)
   size_t hcap = 0;
+  char* xbuf = nullptr;   // exact modes: device-visible coherent host memory the kernels
+  size_t xcap = 0;        // read and write directly (per-level children, heap order)
   std::string err;
   bool profiling = false;
   std::vector<hipEvent_t> pev;  // event pool for per-kernel timing

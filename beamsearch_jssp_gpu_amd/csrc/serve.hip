@@ -326,7 +326,7 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->hdesc.resize(s->nslots);
   for (int k = 0; k < 2; ++k) {
     s->status_ticket[k].assign(s->nslots, 0);
-    if (hipHostMalloc((void**)&s->status[k], (size_t)s->nslots * sizeof(int)) != hipSuccess ||
+    if (hipHostMalloc((void**)&s->status[k], (size_t)s->nslots * sizeof(int), hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&s->status_dev[k], s->status[k], 0) != hipSuccess ||
         hipEventCreateWithFlags(&s->status_ev[k], hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
