@@ -157,3 +157,29 @@ def test_serving_steps_are_pipelined(engine):
     assert srv.step(0) == 0  # no levels: waits and collects
     _same(srv.poll(t2), alone)
     srv.close()
+
+
+def test_large_snapshot_batch_and_serving(engine):
+    """Instances whose snapshot exceeds 128 words or that have more than 64
+    jobs take the large rank-kernel variant (register-limited in the batch
+    and server form); mixed with small ones, each result equals its single
+    solve, in a batch and through the server."""
+    rng = np.random.default_rng(9)
+    jobs = []
+    for t in range(10):
+        n, m = [(70, 5), (100, 8), (60, 10), (20, 10), (30, 15)][t % 5]
+        K = int(rng.choice([64, 256]))
+        mode = [bs.MODE_ICH_DET, bs.MODE_PMS_GPU][t % 2]
+        inst = splitmix_instance(5000 + t, n, m, 1, 99)
+        jobs.append((inst, dict(beam_width=K, mode=mode, phases=2, best_known=bs.NO_MAKESPAN, seed=11 + t)))
+    print("large-snapshot batch of", len(jobs), flush=True)
+    singles = [engine.solve(_b(i), **p) for i, p in jobs]
+    got = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+    for g, s in zip(got, singles):
+        _same(g, s)
+    srv = bs.Server(engine, hbm_budget_bytes=512 << 20, max_slots=16)
+    tickets = [srv.submit(_b(i), p["beam_width"], p["mode"], p["best_known"], p["seed"]) for i, p in jobs]
+    srv.drain()
+    for t, s in zip(tickets, singles):
+        _same(srv.poll(t), s)
+    srv.close()
