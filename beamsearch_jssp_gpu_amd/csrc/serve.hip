@@ -65,6 +65,7 @@ struct jssp_server {
   std::vector<Shape> slot_shape;
   std::vector<Work> slot_work;
   std::vector<Region> slot_region;
+  std::vector<int> slot_level;  // levels enqueued since the slot's request was admitted
   std::deque<uint64_t> queue;
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_ticket = 1;
@@ -159,6 +160,8 @@ int admit(jssp_server* s) {
     jssp_params q = r.params;
     q.phases = 2;
     InstDesc d = jssp::make_desc(shape, w, &q, &r.inst);
+    d.base.jump = w.jump;  // the step that ends the request runs k_jump_b before its backtrack
+    s->slot_level[slot] = 0;
     s->hdesc[slot] = d;
     s->slot_ticket[slot] = s->queue.front();
     s->slot_shape[slot] = shape;
@@ -323,6 +326,7 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->slot_shape.resize(s->nslots);
   s->slot_work.resize(s->nslots);
   s->slot_region.resize(s->nslots);
+  s->slot_level.assign(s->nslots, 0);
   s->hdesc.resize(s->nslots);
   for (int k = 0; k < 2; ++k) {
     s->status_ticket[k].assign(s->nslots, 0);
@@ -420,6 +424,15 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
     const int par = s->parity;
     s->parity ^= 1;
     s->status_ticket[par] = s->slot_ticket;
+    // Requests whose last level runs in this step get their jump table, so
+    // the backtrack walks their trail in parallel 32-level groups.
+    for (int i = 0; i < s->nslots; ++i) {
+      if (!s->slot_ticket[i]) continue;
+      const Shape& sh = s->slot_shape[i];
+      if (s->slot_level[i] < sh.L && s->slot_level[i] + levels >= sh.L)
+        JSSP_HIP(e, jssp::launch_step_jump(s->d_desc + i, sh.K, sh.L, e->stream));
+      s->slot_level[i] = std::min(sh.L, s->slot_level[i] + levels);
+    }
     JSSP_HIP(e, jssp::launch_step_backtrack(s->d_desc, s->d_slots, s->nact, e->stream, s->status_dev[par]));
     JSSP_HIP(e, hipEventRecord(s->status_ev[par], e->stream));
     s->status_pending = par;
