@@ -154,7 +154,11 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int r
   const long long N = (long long)s.K * s.B;
   const int parents_per_block = kScoreWaves * (64 / s.G);
   s.nblk_score = (int)std::max(1, std::min((s.K + parents_per_block - 1) / parents_per_block, 512));
-  s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads, 512ll));
+  // Throughput shapes (batches, the server): at most 64 filter blocks per
+  // instance, each taking several 1024-child tiles, so fewer blocks re-read
+  // the instance's histogram copies.
+  s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads,
+                                               rank_waves == kRankWavesThroughput ? 64ll : 512ll));
   s.rank_waves = rank_waves == kRankWavesThroughput ? kRankWavesThroughput : kRankWaves;
   s.nblk_rank = (int)std::max(1, std::min((s.K + s.rank_waves - 1) / s.rank_waves, 16 * 256 / s.rank_waves));
   return s;
