@@ -23,9 +23,12 @@ Also reported on the same JSON line:
   small_configs    10x5 K=64 BS-ICH (ICH_EXACT) over 32 instances and 20x20
                    K=1024 single-phase PMS_GPU / ICH_DET over 16 instances
   serving          Poisson arrivals into the continuous-batching server vs the
-                   plain loop on the same request list: solves/s, p50/p99;
-                   then every request queued at once (the server's capacity
-                   vs the plain loop's, 1 / mean service time)
+                   plain loop on the same request list: solves/s, p50/p99 at
+                   2x the plain loop's capacity and (load_sweep) at 0.5x, 1x
+                   and 4x; then every request queued at once (the server's
+                   capacity vs the plain loop's, 1 / mean service time)
+  instance_seeds_1_8  the headline solve on the instances of generator seeds
+                   1..8 (states/s over all eight, makespans)
   cpu_baseline     the reference's own BS-ICH (cpu/v0_ich) compiled into
                    oracle/_ref from the reference sources (or the oracle's
                    restatement when that library is absent), one thread, one
@@ -160,18 +163,12 @@ def pct(v, q):
     return v[min(len(v) - 1, max(0, int(round(q * (len(v) - 1)))))]
 
 
-def serving(eng, bs, count, budget_gib, load):
-    reqs, gaps = serve_workload(bs, count)
-    # Plain loop: solve times measured one by one; a FIFO server on one GPU.
-    for inst, K in reqs[:2]:
-        eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345)
-    plain, dur = [], []
-    for inst, K in reqs:
-        t0 = time.perf_counter()
-        plain.append(eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345))
-        dur.append(time.perf_counter() - t0)
+def serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, load):
+    """One Poisson run at `load` x the plain loop's capacity: the plain loop's
+    latencies follow from the measured service times (a FIFO server on one
+    GPU), the server's are measured in real time on the same arrivals."""
+    count = len(reqs)
     mean_service = sum(dur) / len(dur)
-    # Arrival rate: `load` x the plain loop's capacity.
     rate = load / mean_service
     arr, t = [], 0.0
     for g in gaps:
@@ -182,7 +179,6 @@ def serving(eng, bs, count, budget_gib, load):
         fin = max(fin, a) + d
         lat_plain.append(fin - a)
     plain_span = fin - arr[0]
-    # Served: the same arrival times in real time.
     srv = bs.Server(eng, int(budget_gib * 2**30), max_slots=64)
     tickets, done, lat = [None] * count, {}, [0.0] * count
     nxt = 0
@@ -206,6 +202,27 @@ def serving(eng, bs, count, budget_gib, load):
     srv.close()
     eq = all(done[i].makespan == plain[i].makespan and (done[i].sequence == plain[i].sequence).all()
              for i in range(count))
+    return {"load": load, "arrival_rate_per_s": round(rate, 1),
+            "solves_per_s": round(count / served_span, 2), "plain_solves_per_s": round(count / plain_span, 2),
+            "p50_ms": round(pct(lat, 0.5) * 1e3, 2), "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
+            "plain_p50_ms": round(pct(lat_plain, 0.5) * 1e3, 2),
+            "plain_p99_ms": round(pct(lat_plain, 0.99) * 1e3, 2), "identical_results": eq}
+
+
+def serving(eng, bs, count, budget_gib, load, sweep=(0.5, 1.0, 4.0)):
+    reqs, gaps = serve_workload(bs, count)
+    # Plain loop: solve times measured one by one.
+    for inst, K in reqs[:2]:
+        eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345)
+    plain, dur = [], []
+    for inst, K in reqs:
+        t0 = time.perf_counter()
+        plain.append(eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345))
+        dur.append(time.perf_counter() - t0)
+    mean_service = sum(dur) / len(dur)
+    main = serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, load)
+    runs = [serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, x) for x in sweep]
+    eq = main["identical_results"] and all(r["identical_results"] for r in runs)
     # Saturation: every request queued at t=0, so the server's capacity shows
     # rather than the arrival rate (the plain loop's capacity is 1/mean service).
     srv = bs.Server(eng, int(budget_gib * 2**30), max_slots=64)
@@ -218,13 +235,14 @@ def serving(eng, bs, count, budget_gib, load):
     eq = eq and all(sat[i].makespan == plain[i].makespan and (sat[i].sequence == plain[i].sequence).all()
                     for i in range(count))
     return {"workload": f"{count} requests, Poisson arrivals (seed 7) at {load}x the plain loop's capacity "
-                        f"({rate:.1f}/s), n in [10,100], m in [5,20], K in {{256,1024,4096}}, PMS_GPU, "
-                        f"HBM budget {budget_gib} GiB, 64 slots",
-            "solves_per_s": round(count / served_span, 2), "plain_solves_per_s": round(count / plain_span, 2),
-            "p50_ms": round(pct(lat, 0.5) * 1e3, 2), "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
-            "plain_p50_ms": round(pct(lat_plain, 0.5) * 1e3, 2), "plain_p99_ms": round(pct(lat_plain, 0.99) * 1e3, 2),
+                        f"({main['arrival_rate_per_s']}/s), n in [10,100], m in [5,20], K in {{256,1024,4096}}, "
+                        f"PMS_GPU, HBM budget {budget_gib} GiB, 64 slots",
+            "solves_per_s": main["solves_per_s"], "plain_solves_per_s": main["plain_solves_per_s"],
+            "p50_ms": main["p50_ms"], "p99_ms": main["p99_ms"],
+            "plain_p50_ms": main["plain_p50_ms"], "plain_p99_ms": main["plain_p99_ms"],
             "saturated_solves_per_s": round(count / sat_span, 2),
             "saturated_speedup": round((count / sat_span) / (1.0 / mean_service), 2),
+            "load_sweep": runs,
             "identical_results": eq}
 
 
@@ -400,6 +418,18 @@ def main():
                     for k in kinds if k[2] > 0},
     }
     if not args.no_extras:
+        log("headline over instance seeds 1..8")
+        rs8 = []
+        for sd in range(1, 9):
+            i8 = bs.generate_instance(sd, n, m, 1, 99)
+            r8 = eng.solve(i8, K, mode=mode, phases=phases, seed=seed)
+            assert bs.validate_sequence(i8, r8.sequence) == r8.makespan
+            rs8.append(r8)
+        line["instance_seeds_1_8"] = {
+            "workload": f"the headline solve on {n}x{m} instances of generator seeds 1..8",
+            "states_per_s": round(sum(r.expanded_states for r in rs8) / (sum(r.device_ms for r in rs8) / 1e3), 1),
+            "ms_per_solve": round(sum(r.device_ms for r in rs8) / len(rs8), 3),
+            "makespans": [r.makespan for r in rs8]}
         if phases == 2:
             log("single-phase expand kernel")
             line["expand_single_phase"] = roofline_obj(profile_kinds(eng, inst, K, mode, 1, seed)[0], 1, traffic)
