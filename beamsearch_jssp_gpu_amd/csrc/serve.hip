@@ -74,6 +74,7 @@ struct jssp_server {
   // Rank blocks grouped by kernel variant: variant v's blocks are
   // d_mr[mr_off[v], mr_off[v] + mr_cnt[v]).
   int mr_off[jssp::kRankVariants] = {}, mr_cnt[jssp::kRankVariants] = {};
+  bool filter_tp = true;  // filter form of the current maps (see rebuild_maps)
   size_t flds = 0, rlds = 0;
   std::vector<InstDesc> hdesc;
   // Pipelined steps: each step's last kernel writes every resident slot's
@@ -96,6 +97,9 @@ struct jssp_server {
 };
 
 namespace {
+
+// Filter blocks that fill the chip: one per CU.
+constexpr long long kFillBlocks = 256;
 
 bool arena_alloc(jssp_server* s, size_t bytes, Region& out) {
   bytes = (bytes + 255) & ~(size_t)255;
@@ -211,9 +215,27 @@ int rebuild_maps(jssp_server* s) {
   std::vector<BlockRef> ms, mf, mr, ma, mrv[jssp::kRankVariants];
   s->flds = 0;
   s->rlds = 0;
+  // Filter form: the throughput one (at most 64 blocks per request, two
+  // blocks per CU) while the resident requests' blocks fill the chip; with
+  // few requests left, the single-solve one with up to 512 blocks per
+  // request.  Only the filter's own tiling depends on it, so it may change
+  // between levels; the request's descriptor carries its block count.
+  long long tp_blocks = 0;
+  for (int i = 0; i < s->nslots; ++i)
+    if (s->slot_ticket[i]) tp_blocks += s->slot_shape[i].nblk_filter;
+  s->filter_tp = tp_blocks >= kFillBlocks;
+  std::vector<std::pair<int, int>> nbf_changed;  // (slot, filter blocks)
   for (int i = 0; i < s->nslots; ++i) {
     if (!s->slot_ticket[i]) continue;
-    const Shape& sh = s->slot_shape[i];
+    Shape sh = s->slot_shape[i];
+    if (!s->filter_tp) {
+      const long long N = (long long)sh.K * sh.B;
+      sh.nblk_filter = (int)std::max(1ll, std::min((N + jssp::kFilterThreads - 1) / jssp::kFilterThreads, 512ll));
+    }
+    if (s->hdesc[i].nblk_filter != sh.nblk_filter) {
+      s->hdesc[i].nblk_filter = sh.nblk_filter;
+      nbf_changed.push_back({i, sh.nblk_filter});
+    }
     jssp::append_maps(sh, i, ms, mf, mrv[jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params)]);
     ma.push_back(make_int2(i, 0));
     const Request& r = s->reqs[s->slot_ticket[i]];
@@ -231,7 +253,8 @@ int rebuild_maps(jssp_server* s) {
   s->nact = (int)ma.size();
   if (ms.size() > s->cap_map || mf.size() > s->cap_map || mr.size() > s->cap_map)
     return jssp::set_error(e, JSSP_ERR_LIMIT, "serve", "block map capacity exceeded");
-  const size_t hbytes = (ms.size() + mf.size() + mr.size() + ma.size()) * sizeof(BlockRef);
+  const size_t hbytes = (ms.size() + mf.size() + mr.size() + ma.size()) * sizeof(BlockRef) +
+                        nbf_changed.size() * sizeof(int);
   int rc = jssp::ensure_host_staging(e, hbytes + 64);
   if (rc) return rc;
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
@@ -246,6 +269,12 @@ int rebuild_maps(jssp_server* s) {
   };
   if ((rc = put(s->d_ms, ms)) || (rc = put(s->d_mf, mf)) || (rc = put(s->d_mr, mr)) || (rc = put(s->d_slots, ma)))
     return rc;
+  for (const auto& c : nbf_changed) {
+    std::memcpy(h + off, &c.second, sizeof(int));
+    JSSP_HIP(e, hipMemcpyAsync(&s->d_desc[c.first].nblk_filter, h + off, sizeof(int), hipMemcpyHostToDevice,
+                               e->stream));
+    off += sizeof(int);
+  }
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
   s->maps_dirty = false;
   return JSSP_OK;
@@ -413,7 +442,7 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
   if (s->nact > 0) {
     for (int t = 0; t < levels; ++t) {
       if (s->nms > 0) JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, false, e->stream));
-      JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, true, e->stream));
+      JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, s->filter_tp, e->stream));
       for (int v = 0; v < jssp::kRankVariants; ++v)
         if (s->mr_cnt[v] > 0)
           JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds, v,
