@@ -990,7 +990,10 @@ __device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bu
 }
 
 // Threshold bin from the histogram, block-aggregated compaction of the keys
-// below it (sure survivors) into cand[0, cum_lt).  When the threshold bin
+// below it (sure survivors): into cand[0, cum_lt) with one counter, or, when
+// the threshold bin holds one score value (and K <= kRankLdsMax, B >= 8),
+// into kSureSubs regions of cand, one counter each (block b takes counter
+// b % kSureSubs), which the rank kernel gathers.  When the threshold bin
 // holds one score value, its survivors are its `need` smallest child indices:
 // each tile (one block pass of kFilterThreads children) writes its keys of
 // that bin in index order at the tile's offset in ref_buf and their count in
