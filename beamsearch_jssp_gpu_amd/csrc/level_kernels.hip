@@ -1254,9 +1254,11 @@ __device__ __forceinline__ uint32_t count_below(P C, int n, u64 key, int lane) {
 
 // Rank of every survivor among the Kp survivors, then materialise it
 // (two-phase) or copy it from the child buffer (single-phase) into slot q of
-// the next beam.  One wave per survivor.  cand[0, sf) is staged in LDS and
-// each survivor's rank is the count of smaller keys (exact (score bits, child
-// index) order -- the order CUB's stable sort gives the reference).
+// the next beam.  One wave per survivor.  The sf keys below the threshold
+// bin (cand[0, sf), or the filter's kSureSubs regions gathered in order) are
+// staged in LDS and each survivor's rank is the count of smaller keys (exact
+// (score bits, child index) order -- the order CUB's stable sort gives the
+// reference).
 // Survivors u >= sf are the threshold bin's ties, which sort by child index
 // after every key of cand: survivor u is the (u - sf)-th tie, at rank u.
 // MODE: 0 = the instance's mode at run time (separate score kernel), 1 =
@@ -1264,7 +1266,8 @@ __device__ __forceinline__ uint32_t count_below(P C, int n, u64 key, int lane) {
 // materialising a survivor, the wave writes its words to its LDS view and
 // scores its children for the next level.
 // SMALL (fused only): S <= 128 and n <= 64, so a snapshot is two words per
-// lane and a job one slot per lane.
+// lane and a job one slot per lane.  W: waves per block (kRankWaves for
+// single solves, kRankWavesThroughput for batches and the server).
 template <bool FUSED, int MODE, bool SMALL, int W>
 __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nblk) {
   constexpr int NWM = SMALL ? 2 : kMaxWPL;
