@@ -342,6 +342,8 @@ def main():
     ap.add_argument("--phases", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="headline and roofline only")
+    ap.add_argument("--serving-only", action="store_true",
+                    help="only the serving measurement (its JSON object), e.g. under a profiler")
     ap.add_argument("--cpu-beam", type=int, default=2048)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--requests", type=int, default=64)
@@ -368,6 +370,12 @@ def main():
         sys.exit(2)
 
     import beamsearch_jssp_gpu_amd as bs
+    if args.serving_only:
+        eng = bs.Engine(0)
+        out = serving(eng, bs, args.requests, args.budget_gib, args.serve_load)
+        eng.close()
+        print(json.dumps({"serving": out}), flush=True)
+        return
     n, m, K, seed = args.jobs, args.machines, args.beam, 12345
     inst = bs.generate_instance(1, n, m, 1, 99)
     eng = bs.Engine(0)
