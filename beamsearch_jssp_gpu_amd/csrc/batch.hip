@@ -103,7 +103,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
   std::vector<int> ends;
   for (int i = 0; i < count; ++i) {
     ps[i].phases = 2;  // batches materialise survivors only; beams are identical either way
-    Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0, kRankWavesThroughput);
+    Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0, kRankWavesThroughput, kRankBatch);
     bp.shapes.push_back(s);
     Work w;
     work_need[i] = carve(s, nullptr, w);
@@ -229,7 +229,8 @@ int enqueue_step(jssp_engine* e, const BatchPlan& bp, const Segment& sg) {
   JSSP_HIP(e, launch_step_filter(bp.d_desc, sg.d_mf, sg.nmf, sg.flds, true, e->stream));
   for (int v = 0; v < kRankVariants; ++v)
     if (sg.nmr[v] > 0)
-      JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, kRankWavesThroughput, e->stream));
+      JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, kRankWavesThroughput, e->stream,
+                                    nullptr, kRankBatch));
   return JSSP_OK;
 }
 

@@ -137,7 +137,7 @@ bool fused_eligible(const jssp_instance* in, const jssp_params* p) {
          exact_digit_applies(in);
 }
 
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves) {
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves, int rank_batch) {
   Shape s;
   s.n = in->n_jobs; s.m = in->n_machines;
   s.S = 2 * s.n + s.m + 1;
@@ -160,7 +160,8 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int r
   s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads,
                                                rank_waves == kRankWavesThroughput ? 64ll : 512ll));
   s.rank_waves = rank_waves == kRankWavesThroughput ? kRankWavesThroughput : kRankWaves;
-  s.nblk_rank = (int)std::max(1, std::min((s.K + s.rank_waves - 1) / s.rank_waves, 16 * 256 / s.rank_waves));
+  const int per_blk = s.rank_waves * std::max(1, std::min(rank_batch, kRankBatch));
+  s.nblk_rank = (int)std::max(1, std::min((s.K + per_blk - 1) / per_blk, 16 * 256 / s.rank_waves));
   return s;
 }
 

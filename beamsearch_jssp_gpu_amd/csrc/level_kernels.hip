@@ -1252,6 +1252,38 @@ __device__ __forceinline__ uint32_t count_below(P C, int n, u64 key, int lane) {
   return c;
 }
 
+// The same for NB wave-uniform keys at once (LDS keys): every LDS read is
+// compared against all NB keys, so a wave ranking several survivors reads
+// the staged keys once per NB of them.
+template <int NB>
+__device__ __forceinline__ void count_below_n(const u64* C, int n, const u64 (&key)[NB], uint32_t (&c)[NB],
+                                              int lane) {
+  u64 k[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)key[j]);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(key[j] >> 32));
+    k[j] = ((u64)hi << 32) | lo;
+    c[j] = 0;
+  }
+  int base = 0;
+  for (; base + 512 <= n; base += 512) {
+    u64 v[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) v[t] = C[base + 64 * t + lane];
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) c[j] += (uint32_t)__popcll(__ballot(v[t] < k[j]));
+  }
+  for (; base < n; base += 64) {
+    const int i = base + lane;
+    const u64 v = i < n ? C[i] : ~0ull;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) c[j] += (uint32_t)__popcll(__ballot(i < n && v < k[j]));
+  }
+}
+
 // Rank of every survivor among the Kp survivors, then materialise it
 // (two-phase) or copy it from the child buffer (single-phase) into slot q of
 // the next beam.  One wave per survivor.  The sf keys below the threshold
@@ -1268,7 +1300,7 @@ __device__ __forceinline__ uint32_t count_below(P C, int n, u64 key, int lane) {
 // SMALL (fused only): S <= 128 and n <= 64, so a snapshot is two words per
 // lane and a job one slot per lane.  W: waves per block (kRankWaves for
 // single solves, kRankWavesThroughput for batches and the server).
-template <bool FUSED, int MODE, bool SMALL, int W>
+template <bool FUSED, int MODE, bool SMALL, int W, int NB>
 __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nblk) {
   constexpr int NWM = SMALL ? 2 : kMaxWPL;
   constexpr int NHM = SMALL ? 1 : 2;
@@ -1380,11 +1412,35 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   const int nwaves = W;
   const bool single = !FUSED && a.cbuf != nullptr;
   const int order = MODE == 1 ? kOrderErase : (MODE == 2 ? kOrderSwap : ((a.mode == 1) ? kOrderErase : kOrderSwap));
-  for (int u = local * nwaves + wave; u < Kp; u += nblk * nwaves) {
+  // A wave takes survivors u0, u0 + stride, u0 + 2*stride, ...; with NB > 1
+  // (batches), when it holds two or more of the sorted ones, NB of them at a
+  // time are ranked together from LDS, then materialised one by one in that
+  // order.
+  const int stride = nblk * nwaves;
+  for (int ub = local * nwaves + wave; ub < Kp; ub += NB * stride) {
+    u64 bkey[NB] = {};
+    uint32_t bcnt[NB] = {};
+    const bool bat = NB > 1 && in_lds && ub + stride < sf;
+    if constexpr (NB > 1) {
+      if (bat) {
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+          const int u = ub + j * stride;
+          bkey[j] = u < sf ? s_c[u] : ~0ull;
+        }
+        count_below_n<NB>(s_c, sf, bkey, bcnt, lane);
+      }
+    }
+  for (int j = 0; j < NB; ++j) {
+    const int u = ub + j * stride;
+    if (u >= Kp) break;
     uint32_t cnt = (uint32_t)u;
     u64 key;
     if (u < sf) {
-      if (in_lds) {
+      if (bat) {
+        key = bkey[0];
+        cnt = bcnt[0];
+      } else if (in_lds) {
         key = s_c[u];
         cnt = count_below(s_c, sf, key, lane);
       } else {
@@ -1393,6 +1449,13 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
       }
     } else {
       key = tie_key(gp(a.ref_buf), s_tp, tiles, (uint32_t)(u - sf));
+    }
+    if constexpr (NB > 1) {
+#pragma unroll
+      for (int t = 0; t + 1 < NB; ++t) {
+        bkey[t] = bkey[t + 1];
+        bcnt[t] = bcnt[t + 1];
+      }
     }
     const int q = (int)cnt;
     const uint32_t x = (uint32_t)key;
@@ -1430,6 +1493,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     w_min = min(w_min, cm);
     w_max = max(w_max, hi);
     w_minh = min(w_minh, hi);
+  }
   }
   if (lane == 0) { s_mc[wave] = w_min; s_mh[wave] = w_max; s_ml[wave] = w_minh; }
   __syncthreads();
@@ -1764,7 +1828,7 @@ __global__ __launch_bounds__(kFilterThreads, 8) void k_filter_tp_b(InstDesc* des
 
 // Rank + materialise; instances of the other form return at once (a batch
 // mixing both forms launches both kernels over the same map).
-template <int W>
+template <int W, int NB>
 __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_rank_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
@@ -1775,13 +1839,13 @@ __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_r
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<false, 0, false, W>(a, nullptr, r.y, nblk);
+  rank_body<false, 0, false, W, NB>(a, nullptr, r.y, nblk);
 }
 
 // Rank + materialise + score of the next level (fused form), per mode.
 // Throughput blocks: four per CU for the small variant (64 registers), three
 // for the large one (its spills at 64 cost more than the fourth block gains).
-template <int MODE, bool SMALL, int W>
+template <int MODE, bool SMALL, int W, int NB>
 __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SMALL ? 8 : 6) : 4) void k_rank_fused_b(
     InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
@@ -1796,7 +1860,7 @@ __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SMALL ? 8 : 6)
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<true, MODE, SMALL, W>(a, crec_next, r.y, nblk);
+  rank_body<true, MODE, SMALL, W, NB>(a, crec_next, r.y, nblk);
 }
 
 // Jump table of a single solve (base.jump): thread (g, q) follows survivor
@@ -1874,24 +1938,26 @@ hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, siz
   return hipGetLastError();
 }
 
-template <int W>
+template <int W, int NB>
 static void (*rank_kernel(int variant))(InstDesc*, const BlockRef*) {
   switch (variant) {
-    case kRankFusedIch: return k_rank_fused_b<1, false, W>;
-    case kRankFusedPms: return k_rank_fused_b<2, false, W>;
-    case kRankFusedIchSmall: return k_rank_fused_b<1, true, W>;
-    case kRankFusedPmsSmall: return k_rank_fused_b<2, true, W>;
-    default: return k_rank_b<W>;
+    case kRankFusedIch: return k_rank_fused_b<1, false, W, NB>;
+    case kRankFusedPms: return k_rank_fused_b<2, false, W, NB>;
+    case kRankFusedIchSmall: return k_rank_fused_b<1, true, W, NB>;
+    case kRankFusedPmsSmall: return k_rank_fused_b<2, true, W, NB>;
+    default: return k_rank_b<W, NB>;
   }
 }
 
 hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
-                            int waves, hipStream_t s, const EventPair* ev) {
-  if (waves == kRankWavesThroughput)
-    launch(rank_kernel<kRankWavesThroughput>(variant), dim3(nblk), dim3(64 * kRankWavesThroughput), lds_bytes, s,
-           ev, desc, map);
+                            int waves, hipStream_t s, const EventPair* ev, int rank_batch) {
+  constexpr int TW = kRankWavesThroughput;
+  if (waves == TW && rank_batch > 1)
+    launch(rank_kernel<TW, kRankBatch>(variant), dim3(nblk), dim3(64 * TW), lds_bytes, s, ev, desc, map);
+  else if (waves == TW)
+    launch(rank_kernel<TW, 1>(variant), dim3(nblk), dim3(64 * TW), lds_bytes, s, ev, desc, map);
   else
-    launch(rank_kernel<kRankWaves>(variant), dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
+    launch(rank_kernel<kRankWaves, 1>(variant), dim3(nblk), dim3(64 * kRankWaves), lds_bytes, s, ev, desc, map);
   return hipGetLastError();
 }
 

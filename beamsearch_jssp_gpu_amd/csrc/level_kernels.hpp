@@ -28,6 +28,8 @@ constexpr int kRankWaves = 16;
 // a CU and one block's memory waits overlap another's work (single solves
 // keep one 16-wave block per CU: fewer blocks stage the survivor keys).
 constexpr int kRankWavesThroughput = 8;
+// Survivors a wave of those blocks ranks together (one pass over the LDS keys).
+constexpr int kRankBatch = 4;
 // Survivors ranked in LDS (8 bytes each) up to this beam width; wider beams
 // rank from global memory.
 constexpr int kRankLdsMax = 16384;
@@ -177,9 +179,11 @@ enum RankVariant : int {
   kRankFusedPmsSmall = 4,
   kRankVariants = 5
 };
-// waves: kRankWaves or kRankWavesThroughput (the Shape's rank_waves).
+// waves: kRankWaves or kRankWavesThroughput (the Shape's rank_waves);
+// rank_batch > 1 (throughput waves only): the kernels that rank kRankBatch
+// survivors of a wave together (the Shape's rank_batch).
 hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, int variant,
-                            int waves, hipStream_t s, const EventPair* ev = nullptr);
+                            int waves, hipStream_t s, const EventPair* ev = nullptr, int rank_batch = 1);
 // status (optional, device-visible host memory): per instance of the map,
 // whether it is finished after this launch.
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s,

@@ -63,7 +63,10 @@ bool fused_eligible(const jssp_instance* in, const jssp_params* p);
 // always uses a 64-lane score kernel).
 // rank_waves: kRankWaves (single solves) or kRankWavesThroughput (batches,
 // the server).
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves = kRankWaves);
+// rank_batch: survivors per rank wave at full width (kRankBatch for batches,
+// whose instances fill the chip together; 1 keeps every wave to one survivor).
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves = kRankWaves,
+                 int rank_batch = 1);
 // Bytes of the workspace (base == nullptr) or carve it at base.
 size_t carve(const Shape& s, char* base, Work& w);
 LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p);
