@@ -14,14 +14,14 @@ def _b(i):
     return bs.JSSPInstance(i.machine, i.duration, i.name)
 
 
-def _mixed(seed, count, ns=(10, 15, 20, 30, 50), ms=(5, 10, 15, 20), ks=(32, 64, 128)):
+def _mixed(seed, count, ns=(10, 15, 20, 30, 50), ms=(5, 10, 15, 20), ks=(32, 64, 128), phases=2):
     rng = np.random.default_rng(seed)
     out = []
     for t in range(count):
         n, m, K = int(rng.choice(ns)), int(rng.choice(ms)), int(rng.choice(ks))
         mode = int(rng.choice([bs.MODE_ICH_DET, bs.MODE_PMS_GPU]))
         inst = splitmix_instance(1000 + seed * 100 + t, n, m, 1, 99)
-        out.append((inst, dict(beam_width=K, mode=mode, phases=2, best_known=bs.NO_MAKESPAN, seed=7 + t)))
+        out.append((inst, dict(beam_width=K, mode=mode, phases=phases, best_known=bs.NO_MAKESPAN, seed=7 + t)))
     return out
 
 
@@ -31,9 +31,12 @@ def _same(a, b):
         np.testing.assert_array_equal(a.sequence, b.sequence)
 
 
-def test_batch_equals_single_solves(engine):
-    jobs = _mixed(1, 24)
-    print("batch of", len(jobs), flush=True)
+@pytest.mark.parametrize("phases", [2, 1])
+def test_batch_equals_single_solves(engine, phases):
+    """Uneven instances batched (several survivors per rank wave, ties of
+    ICH_DET included) equal their single solves, two-phase and single-phase."""
+    jobs = _mixed(1, 24, phases=phases)
+    print("batch of", len(jobs), "phases", phases, flush=True)
     singles = [engine.solve(_b(i), **p) for i, p in jobs]
     for use_graph in (True, False):
         got = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs], use_graph=use_graph)
