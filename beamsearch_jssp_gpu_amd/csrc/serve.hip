@@ -72,8 +72,9 @@ struct jssp_server {
   bool maps_dirty = true;
   int nms = 0, nmf = 0, nmr = 0, nact = 0;
   // Rank blocks grouped by kernel variant: variant v's blocks are
-  // d_mr[mr_off[v], mr_off[v] + mr_cnt[v]).
-  int mr_off[jssp::kRankVariants] = {}, mr_cnt[jssp::kRankVariants] = {};
+  // d_mr[mr_off[v], mr_off[v] + mr_cnt[v]); v + kRankVariants: the same
+  // variant's requests in the batched rank form.
+  int mr_off[2 * jssp::kRankVariants] = {}, mr_cnt[2 * jssp::kRankVariants] = {};
   bool filter_tp = true;  // filter form of the current maps (see rebuild_maps)
   size_t flds = 0, rlds = 0;
   std::vector<InstDesc> hdesc;
@@ -100,6 +101,9 @@ namespace {
 
 // Filter blocks that fill the chip: one per CU.
 constexpr long long kFillBlocks = 256;
+// Requests waiting (the admitted one included) from which admission picks
+// the batched rank form.
+constexpr size_t kDeepQueue = 8;
 
 bool arena_alloc(jssp_server* s, size_t bytes, Region& out) {
   bytes = (bytes + 255) & ~(size_t)255;
@@ -131,12 +135,13 @@ void arena_free(jssp_server* s, Region reg) {
   }
 }
 
-// Workspace bytes of a request (two-phase, tuned lanes per parent).
-size_t footprint(jssp_server* s, const Request& r, Shape& shape) {
+// Workspace bytes of a request (two-phase, tuned lanes per parent;
+// rank_batch survivors per rank wave).
+size_t footprint(jssp_server* s, const Request& r, Shape& shape, int rank_batch = 1) {
   jssp_params q = r.params;
   q.phases = 2;
   const int lanes = jssp_engine_tuned_lanes(s->e, r.inst.n_jobs, r.inst.n_machines, q.beam_width);
-  shape = jssp::make_shape(&r.inst, &q, lanes, jssp::kRankWavesThroughput);
+  shape = jssp::make_shape(&r.inst, &q, lanes, jssp::kRankWavesThroughput, rank_batch);
   Work w;
   return jssp::carve(shape, nullptr, w);
 }
@@ -155,8 +160,12 @@ int admit(jssp_server* s) {
     for (int i = 0; i < s->nslots; ++i)
       if (!s->slot_ticket[i]) { slot = i; break; }
     if (slot < 0) break;
+    // A request admitted with kDeepQueue or more waiting takes the batched
+    // rank form (a quarter of the rank blocks, four survivors per wave): the
+    // device is then throughput-bound; with few waiting, every block counts
+    // toward the request's latency.
     Shape shape;
-    const size_t bytes = footprint(s, r, shape);
+    const size_t bytes = footprint(s, r, shape, s->queue.size() >= kDeepQueue ? jssp::kRankBatch : 1);
     Region reg;
     if (!arena_alloc(s, bytes, reg)) break;
     Work w;
@@ -212,7 +221,7 @@ int admit(jssp_server* s) {
 
 int rebuild_maps(jssp_server* s) {
   jssp_engine* e = s->e;
-  std::vector<BlockRef> ms, mf, mr, ma, mrv[jssp::kRankVariants];
+  std::vector<BlockRef> ms, mf, mr, ma, mrv[2 * jssp::kRankVariants];
   s->flds = 0;
   s->rlds = 0;
   // Filter form: the throughput one (at most 64 blocks per request, two
@@ -236,13 +245,14 @@ int rebuild_maps(jssp_server* s) {
       s->hdesc[i].nblk_filter = sh.nblk_filter;
       nbf_changed.push_back({i, sh.nblk_filter});
     }
-    jssp::append_maps(sh, i, ms, mf, mrv[jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params)]);
+    const int rv = jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params);
+    jssp::append_maps(sh, i, ms, mf, mrv[rv + (sh.rank_batch > 1 ? jssp::kRankVariants : 0)]);
     ma.push_back(make_int2(i, 0));
     const Request& r = s->reqs[s->slot_ticket[i]];
     s->flds = std::max(s->flds, jssp::filter_lds_bytes(sh, &r.params));
     s->rlds = std::max(s->rlds, jssp::rank_lds_bytes(sh));
   }
-  for (int v = 0; v < jssp::kRankVariants; ++v) {
+  for (int v = 0; v < 2 * jssp::kRankVariants; ++v) {
     s->mr_off[v] = (int)mr.size();
     s->mr_cnt[v] = (int)mrv[v].size();
     mr.insert(mr.end(), mrv[v].begin(), mrv[v].end());
@@ -443,10 +453,11 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
     for (int t = 0; t < levels; ++t) {
       if (s->nms > 0) JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, false, e->stream));
       JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, s->filter_tp, e->stream));
-      for (int v = 0; v < jssp::kRankVariants; ++v)
+      for (int v = 0; v < 2 * jssp::kRankVariants; ++v)
         if (s->mr_cnt[v] > 0)
-          JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds, v,
-                                             jssp::kRankWavesThroughput, e->stream));
+          JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds,
+                                             v % jssp::kRankVariants, jssp::kRankWavesThroughput, e->stream,
+                                             nullptr, v >= jssp::kRankVariants ? jssp::kRankBatch : 1));
     }
     // This step's status, read by the next serve step (or by this one when
     // it runs no levels).

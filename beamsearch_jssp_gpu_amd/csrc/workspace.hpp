@@ -48,6 +48,7 @@ struct Shape {
   int G;                       // lanes per parent in the score kernel
   bool fused;                  // the rank kernel scores the next level (no score kernel)
   int rank_waves;              // waves per rank block: kRankWaves, or kRankWavesThroughput
+  int rank_batch;              // survivors per rank wave at full width: 1, or kRankBatch
   int nblk_score, nblk_filter, nblk_rank;
 };
 
