@@ -121,6 +121,20 @@ def test_serving_alone_equals_in_mix(engine):
     srv.close()
 
 
+def test_serving_deep_queue_equals_alone(engine):
+    """Requests queued at once: those admitted behind eight or more waiting
+    take the batched rank form, the last ones the per-survivor form; both
+    run in the same steps and every result equals its single solve."""
+    jobs = _mixed(6, 16, ks=(64, 128, 256, 512))
+    alone = [engine.solve(_b(i), **p) for i, p in jobs]
+    srv = bs.Server(engine, hbm_budget_bytes=512 << 20, max_slots=16)
+    tickets = [srv.submit(_b(i), p["beam_width"], p["mode"], p["best_known"], p["seed"]) for i, p in jobs]
+    srv.drain()
+    for t, a in zip(tickets, alone):
+        _same(srv.poll(t), a)
+    srv.close()
+
+
 def test_serving_budget_and_cancel(engine):
     inst = splitmix_instance(5, 20, 10, 1, 99)
     srv = bs.Server(engine, hbm_budget_bytes=4 << 20, max_slots=2)
