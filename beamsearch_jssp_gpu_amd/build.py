@@ -73,9 +73,26 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
           f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + f"{ROCM}/lib"])
     os.replace(tmp, LIB)
+    build_drop_in_driver()
     if verbose:
         print(LIB)
     return LIB
+
+
+REPO = os.path.dirname(HERE)
+DRIVER_SRC = os.path.join(REPO, "tests", "drop_in", "ref_style_driver.cpp")
+DRIVER = os.path.join(REPO, "tests", "drop_in", "build", "ref_style_driver")
+
+
+def build_drop_in_driver() -> None:
+    """Links the reference-style test driver (tests/drop_in) against the
+    library through include/jssp_compat.hpp only, as a reference driver would
+    link; the rpath is relative so it runs from any copy of the tree."""
+    if not os.path.exists(DRIVER_SRC):
+        return
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++11", "-Wall", f"-I{os.path.join(REPO, 'include')}", "-o", DRIVER, DRIVER_SRC,
+          f"-L{LIBDIR}", "-ljssp_hip", "-Wl,-rpath,$ORIGIN/../../../beamsearch_jssp_gpu_amd/lib"])
 
 
 if __name__ == "__main__":

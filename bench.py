@@ -40,7 +40,6 @@ import argparse
 import ctypes
 import json
 import os
-import random
 import sys
 import time
 
@@ -118,14 +117,11 @@ def multiseed(eng, bs, inst, K, seed, beams):
 
 
 def batch_workload(bs, count=256, gen_seed=2026):
-    rng = random.Random(gen_seed)
-    insts, params = [], []
-    for i in range(count):
-        n = rng.choice([10, 15, 20, 30, 50])
-        m = rng.choice([5, 10, 15, 20])
-        K = rng.choice([256, 1024])
-        insts.append(bs.generate_instance(rng.randrange(1, 2**31), n, m, 1, 99))
-        params.append(dict(beam_width=K, mode=bs.MODE_PMS_GPU, phases=2, best_known=bs.NO_MAKESPAN, seed=12345))
+    from beamsearch_jssp_gpu_amd import workloads
+    specs = workloads.batch_specs(count, gen_seed)
+    insts = [bs.generate_instance(s["gen_seed"], s["n"], s["m"], workloads.DMIN, workloads.DMAX) for s in specs]
+    params = [dict(beam_width=s["K"], mode=bs.MODE_PMS_GPU, phases=2, best_known=bs.NO_MAKESPAN, seed=s["seed"])
+              for s in specs]
     return insts, params
 
 
@@ -147,14 +143,10 @@ def batched(eng, bs, count):
 
 
 def serve_workload(bs, count, seed=7):
-    rng = random.Random(seed)
-    reqs = []
-    for i in range(count):
-        n = rng.randint(10, 100)
-        m = rng.randint(5, 20)
-        K = rng.choice([256, 1024, 4096])
-        reqs.append((bs.generate_instance(rng.randrange(1, 2**31), n, m, 1, 99), K))
-    gaps = [rng.expovariate(1.0) for _ in range(count)]  # unit-rate Poisson process, scaled below
+    from beamsearch_jssp_gpu_amd import workloads
+    specs, gaps = workloads.serve_specs(count, seed)  # gaps: unit-rate Poisson process, scaled below
+    reqs = [(bs.generate_instance(s["gen_seed"], s["n"], s["m"], workloads.DMIN, workloads.DMAX), s["K"])
+            for s in specs]
     return reqs, gaps
 
 

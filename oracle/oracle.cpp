@@ -17,9 +17,14 @@
 //                   is min(V, K)).  There is no executable GPU reference here,
 //                   so it is pinned by: brute force (K >= all states), the
 //                   closed-form hash vectors in tests, re-simulation of every
-//                   returned sequence, the (score, child index) total-order
-//                   invariant checked on the returned trail, and agreement of
-//                   its ICH-order child generation with orc_ich_exact's.
+//                   returned sequence, and an independent replay of its
+//                   trail (tests/test_oracle.py) which recomputes every
+//                   child's (score bits, child index) key from the replayed
+//                   parents and requires each level's survivors to be the
+//                   min(V, K) smallest keys in ascending order (the prefix of
+//                   CUB's stable SortPairs, gpu/v1/solver.cu:255-261), the
+//                   expanded count, and the best child's makespan and
+//                   sequence.
 //   orc_simulate    cpu/common/jssp_types.h:75-97 applied to a sequence.
 //   orc_pivot_hash  gpu/v1/solver.cu:51-53; pinned by the published inverse
 //                   of lowbias32.

@@ -1,0 +1,68 @@
+"""The serving loop at its own workload: the 64-request mix of the serving
+measurement (n in [10,100], m in [5,20], K in {256,1024,4096}, PMS_GPU,
+seeded draws of beamsearch_jssp_gpu_amd/workloads.py), arriving in seeded
+staggered bursts between serve steps, with a burst deep enough that later
+requests are admitted behind more than eight waiting ones (the batched rank
+form) while others run the per-survivor form.
+
+Every result must equal the same request solved alone, and the requests
+with K >= 1024 whose oracle results are committed
+(tests/golden/oracle_full_size.json) must equal the oracle."""
+import hashlib
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import beamsearch_jssp_gpu_amd as bs
+from beamsearch_jssp_gpu_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_full_size.json")))
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype="<i4").tobytes()).hexdigest()[:16]
+
+
+def _same(a, b):
+    assert a.makespan == b.makespan and a.found == b.found and a.expanded_states == b.expanded_states
+    if a.found:
+        np.testing.assert_array_equal(a.sequence, b.sequence)
+
+
+def test_serving_mix_equals_alone_and_oracle(engine):
+    specs, _ = W.serve_specs()
+    insts = [bs.generate_instance(s["gen_seed"], s["n"], s["m"], W.DMIN, W.DMAX) for s in specs]
+    print("serving mix: solving", len(specs), "requests alone", flush=True)
+    alone = [engine.solve(i, s["K"], mode=bs.MODE_PMS_GPU, seed=s["seed"]) for i, s in zip(insts, specs)]
+    for g in GOLD["serving"]:
+        assert g["spec"] == specs[g["index"]]
+        r = alone[g["index"]]
+        assert r.makespan == g["makespan"] and r.expanded_states == g["expanded"], g["index"]
+        assert sha(r.sequence) == g["sequence_sha"], g["index"]
+    assert {s["K"] for s in specs} == {256, 1024, 4096}
+    print("serving mix: staggered arrivals", flush=True)
+    srv = bs.Server(engine, hbm_budget_bytes=8 << 30, max_slots=64)
+    rng = random.Random(7)
+    tickets, i = [], 0
+    # first burst: 20 requests at once (admitted behind a deep queue), then
+    # bursts of 0-4 between one to three 16-level steps
+    while i < len(specs):
+        burst = 20 if i == 0 else rng.randint(0, 4)
+        for _ in range(min(burst, len(specs) - i)):
+            tickets.append(srv.submit(insts[i], specs[i]["K"], bs.MODE_PMS_GPU, seed=specs[i]["seed"]))
+            i += 1
+        for _ in range(rng.randint(1, 3)):
+            srv.step(16)
+    srv.drain()
+    for k, (t, a) in enumerate(zip(tickets, alone)):
+        got = srv.poll(t)
+        assert got is not None, k
+        _same(got, a)
+        sub, adm, done = srv.times(t)
+        assert 0 <= sub <= adm <= done
+    srv.close()

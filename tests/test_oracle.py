@@ -9,9 +9,13 @@
   lower bound;
 * the pivot hash against the published inverse of lowbias32 (a wrong
   constant or shift breaks the round trip);
-* orc_gpu_solve's operation-level child generation against orc_ich_exact
-  (same ready order and children; only the score differs), via K=1 greedy
-  equality on instances where the rand noise cannot matter;
+* orc_gpu_solve's survivors, level by level, against an independent Python
+  replay of the GPU semantics (pinned hashes, numpy float32 scores, numpy's
+  stable sort): they must be exactly the min(V, K) smallest (score bits,
+  child index) keys in ascending order; planted convention mistakes (level
+  seed, cid/bid, order, tie order) must make the replay disagree; a level-0
+  known-answer test fixes the noise hash's call convention;
+* orc_gpu_solve's operation-level child count against orc_ich_exact's;
 * the restated glibc generator against the C library's own srand()/rand();
 * the multi-rank driver loops: one rank and one pass reduce to the pinned
   single solves, and the PMS loop equals the reference's own solver run
@@ -86,9 +90,221 @@ def test_sequences_resimulate_and_bound():
             assert r["makespan"] >= lower_bound(inst)
 
 
-def test_gpu_sem_trail_is_sorted_by_score_then_child():
-    """Survivor q of level l comes from parent p with a (score, child) key not
-    above survivor q+1's -- recomputed from the replayed states."""
+def _noise_hash(seed, cid, bid):
+    """gpu/v1/solver.cu:122-125 in Python (pinned below by its inverse)."""
+    h = (seed ^ ((cid * 0x85EBCA6B) & 0xFFFFFFFF) ^ ((bid * 0xC2B2AE35) & 0xFFFFFFFF)) & 0xFFFFFFFF
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & 0xFFFFFFFF
+    h ^= h >> 16
+    h = (h * 0xC2B2AE35) & 0xFFFFFFFF
+    h ^= h >> 16
+    return h
+
+
+def _score_bits(cmax, ns, h):
+    """FP32 fl(fl(cmax / ns) + fl((h % 1000) / 1e9)) as its bit pattern, in
+    numpy's IEEE float32 (gpu/v1/solver.cu:121-127)."""
+    base = np.float32(np.float32(cmax) / np.float32(ns))
+    noise = np.float32(np.float32(h % 1000) / np.float32(1e9))
+    return int(np.float32(base + noise).view(np.uint32))
+
+
+class _Convention:
+    """The call convention of the GPU reference's per-level hashes and
+    selection.  The defaults are the reference's (gpu/v1/solver.cu:51, 122,
+    250, 255-261); the other values are planted mistakes that the replay must
+    catch."""
+
+    def __init__(self, level_seed=True, cid_is_parent=True, descending=False, ties_reversed=False):
+        self.level_seed, self.cid_is_parent = level_seed, cid_is_parent
+        self.descending, self.ties_reversed = descending, ties_reversed
+
+
+def _replay_check(inst, K, mode, best_known, seed, r, conv=_Convention()):
+    """Replays orc_gpu_solve's trail with an independent Python statement of
+    the GPU semantics and returns the first disagreement (None if none).
+
+    At every level it rebuilds the parents from the previous level's
+    survivors, generates every child (ICH_DET: every ready job in ready order;
+    PMS_GPU: the ready jobs on the machine of ready[lowbias32(seed+level+p) %
+    r], gpu/v1/solver.cu:48-69), drops those with cmax >= best_known
+    (:120), keys the rest (FP32 score bits << 32 | p*n + slot, noise
+    hash(seed+level, cid = parent rank p, bid = slot), :121-127) and checks
+    that the trail's survivors are exactly the min(V, K) smallest keys in
+    ascending order -- the prefix of CUB's stable ascending SortPairs
+    (:255-261).  At the last level the returned makespan and sequence must be
+    the complete child minimising (cmax, child index)."""
+    n, m = inst.n, inst.m
+    L, B = n * m, n
+    mach, dur = inst.machine, inst.duration
+    beam = [dict(mr=[0] * m, jr=[0] * n, nx=[0] * n, ready=list(range(n)), cmax=0)]
+    trail, pcount = r["trail"], r["pcount"]
+    expanded = 0
+    best = None
+    for lvl in range(L):
+        lseed = (seed + lvl) & 0xFFFFFFFF if conv.level_seed else seed & 0xFFFFFFFF
+        kids = []
+        for p, st in enumerate(beam):
+            ready = st["ready"]
+            if not ready:
+                continue
+            if mode == ICH_DET:
+                slots = list(ready)
+            else:
+                pj = ready[_lowbias32(lseed + p) % len(ready)]
+                tmc = mach[pj, st["nx"][pj]]
+                slots = [jj for jj in ready if mach[jj, st["nx"][jj]] == tmc]
+            for s, j in enumerate(slots):
+                k = st["nx"][j]
+                mc = int(mach[j, k])
+                pred = st["jr"][j] if m > 1 else (st["jr"][j - 1] if j > 0 else 0)
+                f = max(st["mr"][mc], pred) + int(dur[j, k])
+                cmax = max(st["cmax"], f)
+                expanded += 1
+                if not cmax < best_known:
+                    continue
+                x = p * B + s
+                if lvl == L - 1:
+                    if best is None or (cmax, x) < best[:2]:
+                        best = (cmax, x, p, j)
+                    continue
+                cid, bid = (p, s) if conv.cid_is_parent else (s, p)
+                bits = _score_bits(cmax, lvl + 1, _noise_hash(lseed, cid, bid))
+                kids.append(((bits << 32) | ((0xFFFFFFFF - x) if conv.ties_reversed else x), x, p, j, f, mc))
+        if lvl == L - 1:
+            break
+        # numpy's stable sort of the 64-bit keys (or the mutant's order)
+        keys = np.array([c[0] for c in kids], np.uint64)
+        order = np.argsort(-keys.astype(np.float64) if conv.descending else keys, kind="stable")
+        keep = min(len(kids), K)
+        if pcount[lvl + 1] != keep:
+            return f"level {lvl}: {pcount[lvl + 1]} survivors, want {keep}"
+        nxt = []
+        for q in range(keep):
+            _, x, p, j, f, mc = kids[order[q]]
+            t = int(trail[lvl * K + q])
+            if (t >> 8, t & 0xFF) != (p, j):
+                return f"level {lvl} survivor {q}: (parent, job) {(t >> 8, t & 0xFF)}, want {(p, j)}"
+            st = beam[p]
+            ch = dict(mr=list(st["mr"]), jr=list(st["jr"]), nx=list(st["nx"]), ready=list(st["ready"]),
+                      cmax=max(st["cmax"], f))
+            ch["mr"][mc] = f
+            ch["jr"][j] = f
+            ch["nx"][j] += 1
+            pos = ch["ready"].index(j)
+            if mode == ICH_DET:       # erase, keeping order (cpu/v0_ich/solver.cpp:29)
+                del ch["ready"][pos]
+            else:                     # swap-remove (gpu/v1/solver.cu:101-106)
+                ch["ready"][pos] = ch["ready"][-1]
+                ch["ready"].pop()
+            if ch["nx"][j] < m:       # append the job's next op (gpu_snapshot.cuh:67-69)
+                ch["ready"].append(j)
+            nxt.append(ch)
+        beam = nxt
+    if expanded != r["expanded"]:
+        return f"expanded {r['expanded']}, want {expanded}"
+    if best is None:
+        return None if (not r["found"] and r["makespan"] == best_known) else "found a child where none exists"
+    if not r["found"] or r["makespan"] != min(best[0], best_known):
+        return f"makespan {r['makespan']}, want {best[0]}"
+    # the sequence ends with the best child's job; the rest is its trail walk
+    jobs, q = [best[3]], best[2]
+    for lvl in range(L - 2, -1, -1):
+        t = int(trail[lvl * K + q])
+        jobs.append(t & 0xFF)
+        q = t >> 8
+    jobs.reverse()
+    cnt = [0] * n
+    want = []
+    for j in jobs:
+        want.append(j * m + cnt[j] + 1)
+        cnt[j] += 1
+    if list(r["sequence"]) != want:
+        return "sequence differs from the best child's trail walk"
+    return None
+
+
+# (instance, K, mode, best_known, seed): wide and narrow beams, pruning, m = 1
+_REPLAY_CASES = [
+    (splitmix_instance(3, 8, 5, 1, 99), 16, ICH_DET, NO, 42),
+    (splitmix_instance(3, 8, 5, 1, 99), 16, PMS_GPU, NO, 42),
+    (splitmix_instance(11, 10, 5, 5, 99), 64, ICH_DET, NO, 12345),
+    (splitmix_instance(11, 10, 5, 5, 99), 64, PMS_GPU, NO, 12345),
+    (splitmix_instance(12, 6, 6, 1, 9), 500, ICH_DET, NO, 7),          # equal-score runs
+    (splitmix_instance(12, 6, 6, 1, 9), 500, PMS_GPU, NO, 2**32 - 3),  # level seed wraps
+    (splitmix_instance(13, 12, 4, 1, 99), 24, PMS_GPU, 0, 99),         # bound set below
+    (splitmix_instance(14, 7, 1, 1, 50), 8, ICH_DET, NO, 5),            # m = 1 predecessor rule
+    (splitmix_instance(15, 5, 5, 3, 3), 40, ICH_DET, NO, 1),            # every duration equal
+]
+
+
+@pytest.mark.parametrize("case", range(len(_REPLAY_CASES)))
+def test_gpu_sem_survivors_are_the_k_smallest_keys_in_order(case):
+    """orc_gpu_solve's survivors at every level are exactly the min(V, K)
+    smallest (score bits, child index) keys, ascending, recomputed from the
+    replayed parents with the pinned hashes and numpy float32 scores."""
+    inst, K, mode, bk, seed = _REPLAY_CASES[case]
+    if bk == 0:  # a bound that prunes: just above the unpruned makespan's half-way cmax
+        free = orc_gpu(inst, K, mode, NO, seed)
+        bk = free["makespan"] - 1
+    r = orc_gpu(inst, K, mode, bk, seed)
+    assert _replay_check(inst, K, mode, bk, seed, r) is None
+
+
+@pytest.mark.parametrize("mutant", [dict(level_seed=False), dict(cid_is_parent=False), dict(descending=True),
+                                    dict(ties_reversed=True)])
+def test_replay_catches_planted_convention_mistakes(mutant):
+    """The replay is sensitive: each planted mistake in the convention (level
+    seed without +level, cid and bid swapped, descending order, equal
+    scores ordered by descending child index) disagrees with the oracle on some replay case."""
+    conv = _Convention(**mutant)
+    caught = False
+    for inst, K, mode, bk, seed in _REPLAY_CASES:
+        if bk == 0:
+            continue
+        r = orc_gpu(inst, K, mode, bk, seed)
+        if _replay_check(inst, K, mode, bk, seed, r, conv) is not None:
+            caught = True
+            break
+    assert caught, mutant
+
+
+def test_noise_hash_level0_known_answers():
+    """Level 0 of ICH_DET from the root: every first op lasts 1, so all n
+    children have cmax 1 and score fl(1 + fl((h % 1000) / 1e9)); their order
+    is fixed by the noise hash with seed = the solve seed (level 0), cid = 0
+    (the root's rank) and bid = the slot (= job).  The expected order is
+    computed here from the pinned Python hash and compared with the oracle's
+    level-0 trail."""
+    n, m = 32, 3
+    mach = np.array([[j % m, (j + 1) % m, (j + 2) % m] for j in range(n)], np.int32)
+    dur = np.full((n, m), 5, np.int32)
+    dur[:, 0] = 1
+    from testlib import Inst
+    inst = Inst(mach, dur)
+    for seed in (0, 1, 12345, 2**31 + 17, 2**32 - 1):
+        want = sorted(range(n), key=lambda s: (_score_bits(1, 1, _noise_hash(seed, 0, s)), s))
+        # the KAT distinguishes the convention: several distinct score values
+        assert len({_score_bits(1, 1, _noise_hash(seed, 0, s)) for s in range(n)}) > 3
+        r = orc_gpu(inst, n, ICH_DET, NO, seed)
+        got = [int(r["trail"][q]) for q in range(n)]
+        assert [t >> 8 for t in got] == [0] * n
+        assert [t & 0xFF for t in got] == want, seed
+
+
+def test_python_noise_hash_matches_oracle_and_inverts():
+    lib = oracle()
+    rng = np.random.default_rng(9)
+    for _ in range(300):
+        seed, cid, bid = (int(v) for v in rng.integers(0, 2**32, 3))
+        bid %= 128
+        h = _noise_hash(seed, cid, bid)
+        assert h == lib.orc_noise_hash(seed, cid, bid)
+        assert _unmix_noise(h) == (seed ^ ((cid * 0x85EBCA6B) & 0xFFFFFFFF) ^ ((bid * 0xC2B2AE35) & 0xFFFFFFFF))
+
+
+def test_gpu_sem_trail_parents_in_range():
+    """Every trail entry names a parent of the previous level and a job."""
     inst = splitmix_instance(3, 8, 5, 1, 99)
     K = 16
     for mode in (ICH_DET, PMS_GPU):
@@ -97,7 +313,6 @@ def test_gpu_sem_trail_is_sorted_by_score_then_child():
         assert r["pcount"][0] == 1
         for lvl in range(L - 1):
             assert 0 < r["pcount"][lvl + 1] <= K
-        # parent ranks of consecutive survivors come from the previous level
         for lvl, q, p, j in valid_trail_entries(r["trail"], r["pcount"], K):
             assert 0 <= p < r["pcount"][lvl]
             assert 0 <= j < inst.n
