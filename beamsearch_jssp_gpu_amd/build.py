@@ -82,17 +82,21 @@ def build(verbose: bool = False) -> str:
 REPO = os.path.dirname(HERE)
 DRIVER_SRC = os.path.join(REPO, "tests", "drop_in", "ref_style_driver.cpp")
 DRIVER = os.path.join(REPO, "tests", "drop_in", "build", "ref_style_driver")
+RAND_CHECK_SRC = os.path.join(REPO, "tests", "drop_in", "rand_stream_check.cpp")
+RAND_CHECK = os.path.join(REPO, "tests", "drop_in", "build", "rand_stream_check")
 
 
 def build_drop_in_driver() -> None:
     """Links the reference-style test driver (tests/drop_in) against the
     library through include/jssp_compat.hpp only, as a reference driver would
-    link; the rpath is relative so it runs from any copy of the tree."""
+    link (the rpath is relative so it runs from any copy of the tree), and
+    the host check of the process-wide rand() stream."""
     if not os.path.exists(DRIVER_SRC):
         return
     os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
     _run(["g++", "-O2", "-std=c++11", "-Wall", f"-I{os.path.join(REPO, 'include')}", "-o", DRIVER, DRIVER_SRC,
           f"-L{LIBDIR}", "-ljssp_hip", "-Wl,-rpath,$ORIGIN/../../../beamsearch_jssp_gpu_amd/lib"])
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", RAND_CHECK, RAND_CHECK_SRC])
 
 
 if __name__ == "__main__":

@@ -10,11 +10,13 @@
 
 #include "../../include/jssp.h"
 #include "../../include/jssp_compat.hpp"
+#include "rand_stream.hpp"
 
 namespace {
 
 // One process-wide engine on device 0, created on first use; the reference's
-// solvers are process-global too (c_ops, default stream, rand()).
+// solvers are process-global too (c_ops, default stream, rand()).  Creating
+// it leaves the caller's rand() stream where it was (jssp_engine_create).
 jssp_engine* default_engine() {
   static std::once_flag once;
   static jssp_engine* eng = nullptr;
@@ -114,13 +116,17 @@ int beam_search_pms(const JSSPInstance& inst, int beam_width, int best_known) {
   return run(inst, beam_width, JSSP_MODE_PMS_EXACT, 2, best_known, 0, JSSP_FLAG_GLOBAL_RAND, nullptr);
 }
 
+// The GPU solvers draw no rand() values (gpu/v1/solver.cu:178-298): the
+// caller's stream is left as it was, whatever the device runtime does.
 int run_gpu_beam_search_v1(const JSSPInstance& inst, int beam_width, int best_known, int random_seed,
                            int* out_best_sequence) {
+  jssp::ProcessRandGuard keep_rand;
   return run(inst, beam_width, JSSP_MODE_PMS_GPU, 1, best_known, random_seed, 0, out_best_sequence);
 }
 
 int run_gpu_beam_search_v2(const JSSPInstance& inst, int beam_width, int best_known, int random_seed,
                            int* out_best_sequence) {
+  jssp::ProcessRandGuard keep_rand;
   return run(inst, beam_width, JSSP_MODE_PMS_GPU, 2, best_known, random_seed, 0, out_best_sequence);
 }
 
@@ -137,6 +143,7 @@ int hybrid(const JSSPInstance& inst, int gpu_beam_width, int cpu_beam_width, int
   jssp_engine* e = default_engine();
   Flat f = flatten(inst);
   const uint32_t gpu_seed = (uint32_t)std::rand();  // hybrid/v1/solver.cu:28, thread 0
+  jssp::ProcessRandGuard keep_rand;                  // the caller's stream: that one draw only
   const uint32_t cpu_base = (uint32_t)std::time(nullptr) + (uint32_t)(rank * 1000 + iteration * 100);
   jssp_result r = {};
   int rc = jssp_hybrid_iteration(e, &f.in, gpu_beam_width, cpu_beam_width, best_known, gpu_seed, phases,

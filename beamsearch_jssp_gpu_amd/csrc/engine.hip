@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "rand_stream.hpp"
 #include "level_kernels.hpp"
 #include "snapshot.hpp"
 #include "workspace.hpp"
@@ -630,6 +631,11 @@ extern "C" {
 int jssp_engine_create(int device, jssp_engine** out) {
   if (!out) return JSSP_ERR_INVALID;
   *out = nullptr;
+  // The first use of the device loads the code objects, and the loader
+  // draws from and reseeds the process-wide rand(): keep the caller's stream
+  // where it was (the exact modes continue it, as the reference's CPU
+  // solvers do).
+  jssp::ProcessRandGuard keep_rand;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
     (void)hipGetLastError();

@@ -236,6 +236,7 @@ __global__ void k_init(LevelArgs a) {
     *a.sel = z;
     *a.sel_next = z;
     a.pcount[0] = 1;
+    a.pcount[a.L] = 0;  // no beam after the last level (its filter returns at once)
     *a.expanded = 0ull;
     a.result[0] = a.best_known; a.result[1] = 0; a.result[2] = 0;
   }
@@ -1737,6 +1738,7 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
     *d.sel[0] = z;
     *d.sel[1] = z;
     d.base.pcount[0] = 1;
+    d.base.pcount[d.base.L] = 0;  // no beam after the last level (its filter returns at once)
     *d.base.expanded = 0ull;
     d.base.result[0] = d.base.best_known; d.base.result[1] = 0; d.base.result[2] = 0;
     d.lvl_score = 0;

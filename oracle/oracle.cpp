@@ -384,7 +384,10 @@ int orc_gpu_solve(int n, int m, const int* mach, const int* dur, int beam_width,
   int best_parent = -1, best_job = -1;
   std::vector<int> trail;     // (L-1) x K
   if (L > 1) trail.assign((size_t)(L - 1) * K, -1);
-  if (pcount_out) pcount_out[0] = 1;
+  if (pcount_out) {
+    pcount_out[0] = 1;
+    pcount_out[L] = 0;  // the last level keeps no beam
+  }
   std::vector<Child> kids;
   for (int step = 0; step < L; ++step) {
     const uint32_t lseed = seed + (uint32_t)step;

@@ -50,6 +50,15 @@ def test_driver_links_and_checks_arguments():
     assert out.returncode == 2 and "usage" in out.stderr
 
 
+def test_process_wide_rand_stream_host_check():
+    """The engine's process-wide rand() stream draws the caller's next values
+    and leaves the generator where those rand() calls would, even when other
+    code reseeds or draws from it meanwhile (tests/drop_in/rand_stream_check.cpp)."""
+    exe = os.path.join(REPO, "tests", "drop_in", "build", "rand_stream_check")
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0 and out.stdout.strip() == "ok", out.stderr
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("K,srand_seed,gpu_seed", [(64, 12345, 777), (8, 12345, 3), (256, 12345, 2**31 - 5)])
 def test_drop_in_driver_matches_reference_and_oracle(K, srand_seed, gpu_seed):
