@@ -96,6 +96,8 @@ _SIGNATURES = {
                                   ctypes.POINTER(_Result)]),
     "jssp_solve_trace": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.POINTER(_Params),
                                         ctypes.POINTER(_Result), _I32P, _I32P]),
+    "jssp_solve_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_Instance), ctypes.POINTER(_Params),
+                                          ctypes.POINTER(_Result)]),
     "jssp_solve_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_Instance),
                                         ctypes.POINTER(_Params), ctypes.POINTER(_Result)]),
     "jssp_solve_batch_ex": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(_Instance),
@@ -260,6 +262,20 @@ class Engine:
             self._check(self.lib.jssp_solve(self.h, ctypes.byref(ci), ctypes.byref(pr), ctypes.byref(r)))
         return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
                       int(r.levels), float(r.device_ms), int(r.best_seed_index), trail, pcount)
+
+    def solve_noalloc(self, inst: JSSPInstance, beam_width: int, mode: int = MODE_PMS_GPU, phases: int = 2,
+                      best_known: int = NO_MAKESPAN, seed: int = 0, flags: int = 0) -> Result:
+        """jssp_solve_noalloc: the same solve with no allocation (workspace in
+        the library's static per-device scratch array)."""
+        L = inst.nbjob * inst.nbmachine
+        seq = np.zeros(L, dtype=np.int32)
+        r = _Result()
+        r.sequence = _p32(seq)
+        ci = inst._c()
+        pr = _params(beam_width, mode, phases, best_known, seed, 1, 1, flags)
+        self._check(self.lib.jssp_solve_noalloc(self.h, ctypes.byref(ci), ctypes.byref(pr), ctypes.byref(r)))
+        return Result(int(r.makespan), bool(r.found), seq if r.found else None, int(r.expanded_states),
+                      int(r.levels), float(r.device_ms), int(r.best_seed_index))
 
     def solve_batch(self, insts: Sequence[JSSPInstance], params: Sequence[dict],
                     use_graph: bool = True) -> list[Result]:

@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
-HIP_SOURCES = ["level_kernels.hip", "engine.hip", "batch.hip", "serve.hip", "driver.hip"]
+HIP_SOURCES = ["level_kernels.hip", "engine.hip", "batch.hip", "serve.hip", "driver.hip", "noalloc.hip"]
 CXX_SOURCES = ["host_io.cpp", "compat.cpp", "host_pms.cpp"]
 
 

@@ -355,6 +355,27 @@ void Prof::collect() {
   }
 }
 
+int enqueue_single_solve(jssp_engine* e, const Shape& s, const jssp_params* q, InstDesc* d_desc, Prof* prof) {
+  const size_t flds = filter_lds_bytes(s, q), rlds = rank_lds_bytes(s);
+  const int variant = rank_variant(s, q);
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  JSSP_HIP(e, launch_step_init(d_desc, nullptr, 1, e->stream));
+  for (int l = 0; l < s.L; ++l) {
+    EventPair ep;
+    if (!s.fused)
+      JSSP_HIP(e, launch_step_score(d_desc, nullptr, s.nblk_score, s.single, e->stream,
+                                    prof ? prof->pair(kKindScore, &ep) : nullptr));
+    JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, false, e->stream,
+                                   prof ? prof->pair(kKindFilter, &ep, l < s.L - 1) : nullptr));
+    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, variant, s.rank_waves, e->stream,
+                                 prof ? prof->pair(kKindRank, &ep, l < s.L - 1) : nullptr));
+  }
+  JSSP_HIP(e, launch_step_jump(d_desc, s.K, s.L, e->stream));
+  JSSP_HIP(e, launch_step_backtrack(d_desc, nullptr, 1, e->stream));
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  return JSSP_OK;
+}
+
 namespace {
 
 void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int>& pc,
@@ -405,23 +426,9 @@ int solve_device(jssp_engine* e, const jssp_instance* in, const jssp_params* p, 
   std::memcpy(e->hbuf, &desc, sizeof(InstDesc));
   JSSP_HIP(e, hipMemcpyAsync(d_desc, e->hbuf, sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
-  const size_t flds = filter_lds_bytes(s, &q), rlds = rank_lds_bytes(s);
   Prof prof(e);
-  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
-  JSSP_HIP(e, launch_step_init(d_desc, nullptr, 1, e->stream));
-  for (int l = 0; l < s.L; ++l) {
-    EventPair ep;
-    if (!s.fused)
-      JSSP_HIP(e, launch_step_score(d_desc, nullptr, s.nblk_score, s.single, e->stream,
-                                    prof.pair(kKindScore, &ep)));
-    JSSP_HIP(e, launch_step_filter(d_desc, nullptr, s.nblk_filter, flds, false, e->stream,
-                                   prof.pair(kKindFilter, &ep, l < s.L - 1)));
-    JSSP_HIP(e, launch_step_rank(d_desc, nullptr, s.nblk_rank, rlds, rank_variant(s, &q), s.rank_waves, e->stream,
-                                 prof.pair(kKindRank, &ep, l < s.L - 1)));
-  }
-  JSSP_HIP(e, launch_step_jump(d_desc, s.K, s.L, e->stream));
-  JSSP_HIP(e, launch_step_backtrack(d_desc, nullptr, 1, e->stream));
-  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  rc = enqueue_single_solve(e, s, &q, d_desc, &prof);
+  if (rc) return rc;
   int* h_res = reinterpret_cast<int*>(e->hbuf);
   unsigned long long* h_exp = reinterpret_cast<unsigned long long*>(e->hbuf + 32);
   int* h_seq = reinterpret_cast<int*>(e->hbuf + 64);
@@ -650,6 +657,10 @@ int jssp_engine_create(int device, jssp_engine** out) {
     delete e;
     return JSSP_ERR_HIP;
   }
+  // Device address of the no-allocation entry point's scratch array, taken
+  // once per device with the device current (a failure only disables that
+  // entry point, which then reports it).
+  (void)jssp::noalloc_setup(device);
   *out = e;
   return JSSP_OK;
 }

@@ -74,6 +74,9 @@ void release_batch_state(jssp_engine* e);
 // ICH_EXACT / PMS_EXACT solve drawing the reference's rand() values from rs.
 int solve_exact(jssp_engine* e, const jssp_instance* in, const jssp_params* p, RandStream& rs,
                 jssp_result* res, int32_t* trail_out, int32_t* pcount_out);
+// Takes the device address of the no-allocation entry point's scratch array
+// on `device` (current), once per device (csrc/noalloc.hip).
+hipError_t noalloc_setup(int device);
 // n_seeds beams per pass, n_passes passes; pass e prunes with the best
 // makespan found before it.  Beam i of pass e uses seeds[e * n_seeds + i]
 // (seed + e * n_seeds + i when seeds is null); pass_best[e], when given,

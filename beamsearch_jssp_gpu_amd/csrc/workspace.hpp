@@ -85,6 +85,13 @@ int rank_variant(const Shape& s, const jssp_params* p);
 // Writes (machine, duration) pairs to the staging buffer and copies them to d_ops.
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops);
 
+struct Prof;
+// Enqueues a whole single-beam device solve (ICH_DET / PMS_GPU) on e->stream
+// for a descriptor already in device memory: the init kernel, every level's
+// step kernels, the jump table and the backtrack, between the engine's
+// start and stop events.  prof (may be null) times the step kernels.
+int enqueue_single_solve(jssp_engine* e, const Shape& s, const jssp_params* q, InstDesc* d_desc, Prof* prof);
+
 // Optional per-kernel timing: events recorded by the launches themselves.
 struct Prof {
   jssp_engine* e;

@@ -214,6 +214,22 @@ typedef struct jssp_hybrid_params {
 int jssp_run_hybrid(jssp_engine* engine, const jssp_instance* inst, const jssp_hybrid_params* params,
                     jssp_result* result, int32_t* pass_best);
 
+/* As jssp_solve for one ICH_DET or PMS_GPU beam (n_seeds, n_passes <= 1),
+ * but allocating nothing: no counterpart in the reference.  For callers that
+ * cannot allocate device or pinned host memory per call (tight serving
+ * loops, harnesses that forbid allocation; the engine itself is created
+ * beforehand).  The workspace is the library's static per-device scratch
+ * array of JSSP_NOALLOC_SCRATCH_BYTES (64 MiB, held by every device the
+ * library runs on until the process ends); the instance is copied up from
+ * the caller's arrays and the results come back into `result` (sequence
+ * filled only when found).  JSSP_ERR_LIMIT when the solve's workspace does
+ * not fit, e.g. 50x20 K=4096 fits two-phase (~31 MB) but not single-phase
+ * (~130 MB).  Calls on one device share its array: the library serialises
+ * them (each call is synchronous).  Same results as jssp_solve. */
+#define JSSP_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int jssp_solve_noalloc(jssp_engine* engine, const jssp_instance* inst, const jssp_params* params,
+                       jssp_result* result);
+
 /* Batched solve: advances all `count` instances together, one level per
  * step; a step is three launches (score, filter, rank) whose grids cover every
  * instance, replayed from a captured hipGraph.  Results equal those of
