@@ -102,6 +102,17 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
   }
   return v;
 }
+// Block barrier ordering LDS accesses only.  __syncthreads() is also a
+// workgroup-scope release of global memory, so it waits for every load and
+// store the wave has in flight; this one waits for LDS operations alone, and
+// global loads issued before it keep travelling (their values are waited for
+// where they are used).  Only for barriers that protect LDS data.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Set bits of `mask` below this lane.
 __device__ __forceinline__ uint32_t lanes_below(u64 mask, int lane) {
   return (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
@@ -509,7 +520,7 @@ __device__ __forceinline__ void finish_score_block(ScoreAcc acc, const uint32_t*
   acc.kmin = wave_min_i(acc.kmin);
   acc.best = wave_min64(acc.best);
   if (lane == 0) { s_cnt[wave] = acc.cnt; s_exp[wave] = acc.exp; s_kmin[wave] = acc.kmin; s_best[wave] = acc.best; }
-  __syncthreads();
+  lds_barrier();  // LDS partials and histogram only: the child stores keep draining
   if (threadIdx.x == 0) {
     BlockStat b;
     b.valid = 0; b.expanded = 0; b.min_cmax = INT_MAX; b.pad = 0; b.best = ~0ull;
@@ -1049,7 +1060,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   if (lane == 63) s_w[wave] = inc;
   if (lane == 0) { s_e[wave] = ex; s_km[wave] = km; }
   if (tid == 0) s_T = kNB - 1;
-  __syncthreads();
+  lds_barrier();
   uint32_t wofs = 0, v = 0;
   for (int w = 0; w < 16; ++w) {
     wofs += (w < wave) ? s_w[w] : 0u;
@@ -1076,7 +1087,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   s_ex[2 * tid + 1] = ex0 + h0;
   if (ex0 < Kp && ex0 + h0 >= Kp) s_T = 2 * tid;
   if (ex0 + h0 < Kp && ex0 + h0 + h1 >= Kp) s_T = 2 * tid + 1;
-  __syncthreads();
+  lds_barrier();
   const uint32_t T1 = s_T;
   const uint32_t cum_lt = s_ex[T1];
   const uint32_t t1 = (T1 + 1 < (uint32_t)kNB ? s_ex[T1 + 1] : v) - cum_lt;
@@ -1111,7 +1122,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
       s_off[0][wave] = (uint32_t)__popcll(bs);
       s_off[1][wave] = (uint32_t)__popcll(bt);
     }
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) {
       uint32_t ts = 0, tt = 0;
       for (int w = 0; w < 16; ++w) {
@@ -1124,7 +1135,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
       if (single) gp(a.tcnt)[base / blockDim.x] = tt;
       else s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
     }
-    __syncthreads();
+    lds_barrier();
     if (sure) cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
     if (tie) {
       // Single-valued bin: the tile's ties in child-index order at the
@@ -1133,12 +1144,15 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
       if (single) tiles_keys[base + o] = key;
       else cand[s_base[1] + o] = key;
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (single) return;  // the rank kernel takes the ties tile by tile
 
   // The last block to finish sees every block's candidates and cuts the
   // threshold bin's keys cand[cum_lt, cum_lt + t1) to the `need` survivors.
+  // Every thread's candidate stores complete before the block's ticket (the
+  // tile loop's barriers order LDS only).
+  __syncthreads();
   if (tid == 0) {
     __threadfence();
     const uint32_t ticket = atomicAdd(&sel->done_ctr, 1u);
@@ -1165,23 +1179,34 @@ __device__ __forceinline__ int wave_word(const int (&pw)[kMaxWPL], int o) {
   return __shfl(v, o & 63);
 }
 
-// Builds survivor q of the next beam from its parent p and the child's record
-// (two-phase) and writes the trail entry.  One wave.  Returns the child's cmax;
-// nw[i] = the child's word lane + 64 i.
+// Loads parent p's words into pw[i] = word lane + 64 i (zeros past S).
 // NWM: words per lane the caller supports (S <= 64 * NWM).
-template <int ORDER, int NWM = kMaxWPL>
-__device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout& Ly, int lane, int p, u64 rec,
-                                               int q, int (&nw)[kMaxWPL]) {
-  const int S = Ly.S, m = a.m;
+template <int NWM = kMaxWPL>
+__device__ __forceinline__ void load_parent(const LevelArgs& a, const Layout& Ly, int lane, int p,
+                                            int (&pw)[kMaxWPL]) {
+  const int S = Ly.S;
   const int NW = (S + 63) >> 6;
   GL const int* par = gp(a.beam_cur) + (size_t)p * S;
-  int pw[kMaxWPL];
+#pragma unroll
+  for (int i = 0; i < kMaxWPL; ++i) {
+    const int o = lane + 64 * i;
+    pw[i] = (i < NWM && i < NW && o < S) ? par[o] : 0;
+  }
+}
+
+// Builds survivor q of the next beam from its parent p's words pw and the
+// child's record (two-phase), writes it and the trail entry.  One wave.
+// Returns the child's cmax; nw[i] = the child's word lane + 64 i.
+template <int ORDER, int NWM = kMaxWPL>
+__device__ __forceinline__ int build_child(const LevelArgs& a, const Layout& Ly, int lane, int p, u64 rec, int q,
+                                           const int (&pw)[kMaxWPL], int (&nw)[kMaxWPL]) {
+  const int S = Ly.S, m = a.m;
+  const int NW = (S + 63) >> 6;
   int r = 0;
   int jlast = -1;
 #pragma unroll
   for (int i = 0; i < NWM; ++i) {
     const int o = lane + 64 * i;
-    pw[i] = (i < NW && o < S) ? par[o] : 0;
     const bool isjw = (i < NW) && o >= Ly.off_jw && o < Ly.off_cmax;
     r += __popcll(__ballot(isjw && jw_pos(pw[i]) != kFinishedPos));
   }
@@ -1218,6 +1243,18 @@ __device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout&
   }
   if (lane == 0) gp(a.trail)[(size_t)a.level * a.K + q] = (p << 8) | j;
   return c.cm;
+}
+
+// Builds survivor q of the next beam from its parent p and the child's record
+// (two-phase) and writes the trail entry.  One wave.  Returns the child's cmax;
+// nw[i] = the child's word lane + 64 i.
+// NWM: words per lane the caller supports (S <= 64 * NWM).
+template <int ORDER, int NWM = kMaxWPL>
+__device__ __forceinline__ int materialize_rec(const LevelArgs& a, const Layout& Ly, int lane, int p, u64 rec,
+                                               int q, int (&nw)[kMaxWPL]) {
+  int pw[kMaxWPL];
+  load_parent<NWM>(a, Ly, lane, p, pw);
+  return build_child<ORDER, NWM>(a, Ly, lane, p, rec, q, pw, nw);
 }
 
 // Key of the r-th (0-based) tie in child-index order: tile c holds tcnt[c]
@@ -1381,7 +1418,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     for (int t = t0 + 1; t < t1; ++t) loc += tcnt[t];
     const uint32_t inc = wave_incl_scan(loc, lane);
     if (lane == 63) s_w[wave] = inc;
-    __syncthreads();
+    lds_barrier();
     uint32_t run = inc - loc;
     for (int w = 0; w < wave; ++w) run += s_w[w];
     for (int t = t0; t < t1; ++t) {
@@ -1390,7 +1427,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     }
     if (tid == (int)blockDim.x - 1) s_tp[tiles] = run;
   }
-  __syncthreads();
+  lds_barrier();  // staged keys and tile prefix (LDS)
   const Layout Ly = make_layout(a.n, a.m);
   const int S = Ly.S;
   const int NW = (S + 63) >> 6;
@@ -1437,16 +1474,14 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     if (u >= Kp) break;
     uint32_t cnt = (uint32_t)u;
     u64 key;
+    bool count = false;  // rank by counting the smaller keys (below)
     if (u < sf) {
       if (bat) {
         key = bkey[0];
         cnt = bcnt[0];
-      } else if (in_lds) {
-        key = s_c[u];
-        cnt = count_below(s_c, sf, key, lane);
       } else {
-        key = cand[u];
-        cnt = count_below(cand, sf, key, lane);
+        key = in_lds ? s_c[u] : cand[u];
+        count = true;
       }
     } else {
       key = tie_key(gp(a.ref_buf), s_tp, tiles, (uint32_t)(u - sf));
@@ -1458,11 +1493,26 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
         bcnt[t] = bcnt[t + 1];
       }
     }
-    const int q = (int)cnt;
     const uint32_t x = (uint32_t)key;
     const uint32_t hi = (uint32_t)(key >> 32);
     const int p = (int)(x / (uint32_t)a.B);
-    const u64 rec = crec[x];
+    // Single solves request the child's record and its parent's words
+    // before the count, which then runs (from LDS) while they travel; the
+    // register-limited throughput blocks (batches, the server) cannot hold
+    // the words across the count without spilling, so they load after it.
+    constexpr bool kEarly = (W == kRankWaves && NB == 1);
+    u64 rec = 0;
+    int pw[kMaxWPL];
+    if (kEarly) {
+      rec = crec[x];
+      if (!single) load_parent<NWM>(a, Ly, lane, p, pw);
+    }
+    if (count) cnt = in_lds ? count_below(s_c, sf, key, lane) : count_below(cand, sf, key, lane);
+    if (!kEarly) {
+      rec = crec[x];
+      if (!single) load_parent<NWM>(a, Ly, lane, p, pw);
+    }
+    const int q = (int)cnt;
     int cm;
     if (single) {
       GL const int* src = gp(a.cbuf) + (size_t)x * child_stride(S);
@@ -1472,8 +1522,8 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
       if (lane == 0) gp(a.trail)[(size_t)a.level * a.K + q] = (p << 8) | rec_job(rec);
     } else {
       int nw[kMaxWPL];
-      cm = (order == kOrderErase) ? materialize_rec<kOrderErase, NWM>(a, Ly, lane, p, rec, q, nw)
-                                  : materialize_rec<kOrderSwap, NWM>(a, Ly, lane, p, rec, q, nw);
+      cm = (order == kOrderErase) ? build_child<kOrderErase, NWM>(a, Ly, lane, p, rec, q, pw, nw)
+                                  : build_child<kOrderSwap, NWM>(a, Ly, lane, p, rec, q, pw, nw);
       if (FUSED) {
         // Score the survivor's children for the next level from its words.
         int* vw = s_view + wave * S;
@@ -1497,7 +1547,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   }
   }
   if (lane == 0) { s_mc[wave] = w_min; s_mh[wave] = w_max; s_ml[wave] = w_minh; }
-  __syncthreads();
+  lds_barrier();  // LDS partials: the stores keep draining
   if (tid == 0) {
     RankStat rs;
     rs.min_cmax = INT_MAX;

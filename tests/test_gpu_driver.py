@@ -199,3 +199,51 @@ def test_host_pms_equals_device_pms_exact(engine):
             assert (h.makespan, h.found, h.expanded_states) == (d.makespan, d.found, d.expanded_states), (n, m, K)
             if h.found:
                 np.testing.assert_array_equal(h.sequence, d.sequence)
+
+
+def test_width_study_rows_equal_oracle(engine, tmp_path, capsys):
+    """width_study (cpu/profiling/time_width_study.cpp:20-81): instances la10,
+    la15, la21, la31, one row per width with the columns Width, Instance,
+    Jobs, Machines, Ops, BestSpan, Time(s); width w's ranks seed
+    srand(time + rank + w) and run the PMS pass loop from 500000.  BestSpan
+    equals the oracle's PMS driver loop with those seeds."""
+    import os
+    from beamsearch_jssp_gpu_amd import drivers
+    from testlib import load_json
+    law = load_json("lawrence.json")
+    names = ["la10", "la15", "la21", "la31"]
+    assert [f"la{i}" for i in drivers.WIDTH_STUDY_IDS] == names
+    assert drivers.WIDTH_STUDY_WIDTHS == [100, 200, 400, 800, 1600, 3200, 6400, 12800, 25600]
+    for name in names:
+        d = law[name]
+        with open(tmp_path / f"{name}.txt", "w") as f:
+            f.write(f"{d['n']} {d['m']}\n")
+            for j in range(d["n"]):
+                row = range(j * d["m"], (j + 1) * d["m"])
+                f.write(" ".join(f"{d['machine'][i]} {d['duration'][i]}" for i in row) + "\n")
+    widths, iters, ranks, t = [100, 200], 2, 2, 1717171717
+    cwd = os.getcwd()
+    print("width_study", flush=True)
+    try:
+        assert drivers.main(["width_study", "--dir", str(tmp_path), "--widths", ",".join(map(str, widths)),
+                             "--iterations", str(iters), "--time-seed", str(t), "--ranks", str(ranks)]) == 0
+    finally:
+        os.chdir(cwd)
+    out = capsys.readouterr().out.splitlines()
+    head = [l.split() for l in out if l.startswith("Width")]
+    assert head == [["Width", "Instance", "Jobs", "Machines", "Ops", "BestSpan", "Time(s)"]]
+    rows = [l.split() for l in out if l[:1].isdigit()]
+    assert len(rows) == len(names) * len(widths)
+    insts = golden_instances()
+    k = 0
+    for name in names:
+        for w, K in enumerate(widths):
+            r = rows[k]
+            k += 1
+            d = law[name]
+            assert r[0] == str(K) and r[1] == f"./{name}.txt"
+            assert r[2:5] == [str(d["n"]), str(d["m"]), str(d["n"] * d["m"] + 2)]
+            want = orc_pms_driver(insts[name], K, ranks, iters, t + w)["makespan"]
+            assert int(r[5]) == want, (name, K)
+            assert float(r[6]) >= 0.0
+    assert "Time vs Beam Width study finished." in out[-1]
