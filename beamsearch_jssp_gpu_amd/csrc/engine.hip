@@ -181,6 +181,7 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.cand = b.take<unsigned long long>(N);
   w.ref_buf = b.take<unsigned long long>(N + s.K);
   w.kdig = b.take<uint16_t>(N);
+  w.pvalid = b.take<uint8_t>(s.K);
   w.tcnt = b.take<uint32_t>((N + kFilterThreads - 1) / kFilterThreads + 2);
   w.hist[0] = b.take<uint32_t>((size_t)kNB * kHistCopies);
   w.hist[1] = b.take<uint32_t>((size_t)kNB * kHistCopies);
@@ -239,6 +240,7 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.cand = w.cand;
   a.ref_buf = w.ref_buf;
   a.kdig = w.kdig;
+  a.pvalid = w.pvalid;
   a.tcnt = w.tcnt;
   a.hist = w.hist[0];
   a.hist_next = w.hist[1];

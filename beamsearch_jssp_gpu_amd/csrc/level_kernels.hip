@@ -276,12 +276,14 @@ struct JobView {
 // Where a score pass writes, and the per-level constants it needs.
 // keys: the FP32 score bits of each child slot (the child index is the
 // slot's position: the filter forms (bits << 32 | x) as it reads them), or
-// kNoScore for an invalid or pruned slot.
+// kNoScore for a pruned child; pvalid[p]: the parent's valid slots (the
+// slots past them are not written).
 constexpr uint32_t kNoScore = 0xFFFFFFFFu;  // no finite positive float has these bits
 
 struct ScoreDst {
   GL uint32_t* keys;
   GL uint16_t* kdig;
+  GL uint8_t* pvalid;
   GL u64* crec;
   uint32_t lseed;
   int B, mode, exact_digit, best_known;
@@ -290,6 +292,7 @@ __device__ __forceinline__ ScoreDst score_dst(const LevelArgs& a) {
   ScoreDst d;
   d.keys = reinterpret_cast<GL uint32_t*>(gp(a.keys));
   d.kdig = gp(a.kdig);
+  d.pvalid = gp(a.pvalid);
   d.crec = gp(a.crec);
   d.lseed = a.lseed;
   d.B = a.B;
@@ -500,10 +503,9 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
     }
     lds_count(s_h, d, counted, lane);
   }
-  if (!sl.last && pact) {
-    for (int s = gl; s < B; s += G)
-      if (s >= nvalid) a.keys[(size_t)p * B + s] = kNoScore;
-  }
+  // Slots [nvalid, B) of the parent hold no child: the filter reads the
+  // count instead of a key for each of them.
+  if (!sl.last && pact && gl == 0) a.pvalid[p] = (uint8_t)nvalid;
 }
 
 // Block partials of a score pass: bstat[slot] and the LDS histogram merged
@@ -1031,12 +1033,18 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   // score-block partial per thread (nblk_score <= 512), the level's parents,
   // and this block's first tile of keys and digits (read within the buffers'
   // K*B capacity; slots past this level's count are ignored below).
+  // A slot x = p*B + s holds a key only below its parent's valid count
+  // pvalid[p]; the first tile's keys are read speculatively and masked.
   GL const uint32_t* keys = reinterpret_cast<GL const uint32_t*>(gp(a.keys));
   GL const uint16_t* kdig = gp(a.kdig);
-  const uint32_t cap = (uint32_t)a.K * (uint32_t)a.B;
+  GL const uint8_t* pvalid = gp(a.pvalid);
+  const uint32_t B = (uint32_t)a.B;
+  const uint32_t cap = (uint32_t)a.K * B;
   const uint32_t xf = (uint32_t)local * blockDim.x + tid;
   const uint32_t key_first = xf < cap ? keys[xf] : kNoScore;
   const uint32_t dg_first = xf < cap ? (uint32_t)kdig[xf] : (uint32_t)kNB;
+  const uint32_t pf = xf / B;
+  const uint32_t pv_first = xf < cap ? (uint32_t)pvalid[pf] : 0u;
   uint32_t h0 = 0, h1 = 0;
 #pragma unroll
   for (int c = 0; c < kHistCopies; ++c) {
@@ -1109,11 +1117,13 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
     const bool first = base == (uint32_t)local * blockDim.x;
-    // Both loads at once (the first tile's were issued in the prologue): the
-    // digit of an invalid slot is never read.
-    const uint32_t sb = inb ? (first ? key_first : keys[x]) : kNoScore;
+    // The slot's parent count, then its key and digit (the first tile's were
+    // issued in the prologue): invalid slots read neither.
+    const uint32_t px = x / B;
+    const bool has = inb && (x - px * B) < (first ? pv_first : (uint32_t)pvalid[px]);
+    const uint32_t sb = has ? (first ? key_first : keys[x]) : kNoScore;
     const u64 key = sb != kNoScore ? (((u64)sb << 32) | x) : ~0ull;
-    const uint32_t dg = inb ? (first ? dg_first : (uint32_t)kdig[x]) : (uint32_t)kNB;
+    const uint32_t dg = has ? (first ? dg_first : (uint32_t)kdig[x]) : (uint32_t)kNB;
     const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;
     const bool tie = d == T1;

@@ -89,6 +89,7 @@ struct LevelArgs {
   unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
   unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut / the ties of each filter tile
   uint16_t* kdig;            // K*B: first selection digit of each key
+  uint8_t* pvalid;           // K: valid child slots of each parent (slots >= it hold no key)
   uint32_t* tcnt;            // ceil(K*B/kFilterThreads): keys of that bin per filter tile
   uint32_t* hist;            // kHistCopies x kNB, this level
   uint32_t* hist_next;       // kHistCopies x kNB, zeroed for the next level

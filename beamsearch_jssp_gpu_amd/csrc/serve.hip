@@ -70,11 +70,19 @@ struct jssp_server {
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_ticket = 1;
   bool maps_dirty = true;
-  int nms = 0, nmf = 0, nmr = 0, nact = 0;
-  // Rank blocks grouped by kernel variant: variant v's blocks are
-  // d_mr[mr_off[v], mr_off[v] + mr_cnt[v]); v + kRankVariants: the same
-  // variant's requests in the batched rank form.
-  int mr_off[2 * jssp::kRankVariants] = {}, mr_cnt[2 * jssp::kRankVariants] = {};
+  int nmr = 0, nact = 0;
+  // Resident requests are split into kLanes groups of about equal work, each
+  // advanced on its own stream (lane 0 is the engine's stream): requests are
+  // independent, so one group's launch gaps and tails overlap the other's
+  // kernels.  Group g's score and filter blocks are d_ms[ms_off[g], +
+  // ms_cnt[g]) and d_mf[mf_off[g], + mf_cnt[g]); its rank blocks of kernel
+  // variant v are d_mr[mr_off[g][v], + mr_cnt[g][v]), v + kRankVariants being
+  // the same variant's requests in the batched rank form.
+  static constexpr int kLanes = 2;
+  hipStream_t lane_stream[kLanes] = {};
+  hipEvent_t lane_ev[kLanes] = {};
+  int ms_off[kLanes] = {}, ms_cnt[kLanes] = {}, mf_off[kLanes] = {}, mf_cnt[kLanes] = {};
+  int mr_off[kLanes][2 * jssp::kRankVariants] = {}, mr_cnt[kLanes][2 * jssp::kRankVariants] = {};
   bool filter_tp = true;  // filter form of the current maps (see rebuild_maps)
   size_t flds = 0, rlds = 0;
   std::vector<InstDesc> hdesc;
@@ -221,7 +229,9 @@ int admit(jssp_server* s) {
 
 int rebuild_maps(jssp_server* s) {
   jssp_engine* e = s->e;
-  std::vector<BlockRef> ms, mf, mr, ma, mrv[2 * jssp::kRankVariants];
+  constexpr int G = jssp_server::kLanes;
+  std::vector<BlockRef> ms, mf, mr, ma;
+  std::vector<BlockRef> msg[G], mfg[G], mrv[G][2 * jssp::kRankVariants];
   s->flds = 0;
   s->rlds = 0;
   // Filter form: the throughput one (at most 64 blocks per request, two
@@ -233,6 +243,21 @@ int rebuild_maps(jssp_server* s) {
   for (int i = 0; i < s->nslots; ++i)
     if (s->slot_ticket[i]) tp_blocks += s->slot_shape[i].nblk_filter;
   s->filter_tp = tp_blocks >= kFillBlocks;
+  // Groups: largest requests first, each to the group with the least work
+  // so far (work per level ~ survivors x snapshot words).
+  std::vector<int> order;
+  for (int i = 0; i < s->nslots; ++i)
+    if (s->slot_ticket[i]) order.push_back(i);
+  auto work = [&](int i) { return (long long)s->slot_shape[i].K * s->slot_shape[i].S; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+  long long load[G] = {};
+  std::vector<int> group(s->nslots, 0);
+  for (int i : order) {
+    int g = 0;
+    for (int h = 1; h < G; ++h) g = load[h] < load[g] ? h : g;
+    group[i] = g;
+    load[g] += work(i);
+  }
   std::vector<std::pair<int, int>> nbf_changed;  // (slot, filter blocks)
   for (int i = 0; i < s->nslots; ++i) {
     if (!s->slot_ticket[i]) continue;
@@ -245,20 +270,27 @@ int rebuild_maps(jssp_server* s) {
       s->hdesc[i].nblk_filter = sh.nblk_filter;
       nbf_changed.push_back({i, sh.nblk_filter});
     }
+    const int g = group[i];
     const int rv = jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params);
-    jssp::append_maps(sh, i, ms, mf, mrv[rv + (sh.rank_batch > 1 ? jssp::kRankVariants : 0)]);
+    jssp::append_maps(sh, i, msg[g], mfg[g], mrv[g][rv + (sh.rank_batch > 1 ? jssp::kRankVariants : 0)]);
     ma.push_back(make_int2(i, 0));
     const Request& r = s->reqs[s->slot_ticket[i]];
     s->flds = std::max(s->flds, jssp::filter_lds_bytes(sh, &r.params));
     s->rlds = std::max(s->rlds, jssp::rank_lds_bytes(sh));
   }
-  for (int v = 0; v < 2 * jssp::kRankVariants; ++v) {
-    s->mr_off[v] = (int)mr.size();
-    s->mr_cnt[v] = (int)mrv[v].size();
-    mr.insert(mr.end(), mrv[v].begin(), mrv[v].end());
+  for (int g = 0; g < G; ++g) {
+    s->ms_off[g] = (int)ms.size();
+    s->ms_cnt[g] = (int)msg[g].size();
+    ms.insert(ms.end(), msg[g].begin(), msg[g].end());
+    s->mf_off[g] = (int)mf.size();
+    s->mf_cnt[g] = (int)mfg[g].size();
+    mf.insert(mf.end(), mfg[g].begin(), mfg[g].end());
+    for (int v = 0; v < 2 * jssp::kRankVariants; ++v) {
+      s->mr_off[g][v] = (int)mr.size();
+      s->mr_cnt[g][v] = (int)mrv[g][v].size();
+      mr.insert(mr.end(), mrv[g][v].begin(), mrv[g][v].end());
+    }
   }
-  s->nms = (int)ms.size();
-  s->nmf = (int)mf.size();
   s->nmr = (int)mr.size();
   s->nact = (int)ma.size();
   if (ms.size() > s->cap_map || mf.size() > s->cap_map || mr.size() > s->cap_map)
@@ -382,6 +414,15 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
     jssp_server_destroy(s);
     return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot create the copy stream");
   }
+  s->lane_stream[0] = e->stream;
+  for (int g = 0; g < jssp_server::kLanes; ++g) {
+    if ((g > 0 && hipStreamCreateWithFlags(&s->lane_stream[g], hipStreamNonBlocking) != hipSuccess) ||
+        hipEventCreateWithFlags(&s->lane_ev[g], hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      jssp_server_destroy(s);
+      return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot create the lane streams");
+    }
+  }
   *out = s;
   return JSSP_OK;
 }
@@ -390,6 +431,13 @@ void jssp_server_destroy(jssp_server* s) {
   if (!s) return;
   std::lock_guard<std::recursive_mutex> lk(s->e->mu);
   (void)hipStreamSynchronize(s->e->stream);
+  for (int g = 1; g < jssp_server::kLanes; ++g)
+    if (s->lane_stream[g]) {
+      (void)hipStreamSynchronize(s->lane_stream[g]);
+      (void)hipStreamDestroy(s->lane_stream[g]);
+    }
+  for (int g = 0; g < jssp_server::kLanes; ++g)
+    if (s->lane_ev[g]) (void)hipEventDestroy(s->lane_ev[g]);
   if (s->copy_stream) {
     (void)hipStreamSynchronize(s->copy_stream);
     (void)hipStreamDestroy(s->copy_stream);
@@ -450,14 +498,31 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
   const int prev = s->status_pending;
   s->status_pending = -1;
   if (s->nact > 0) {
+    // Every lane starts after what the engine's stream holds (admission,
+    // the previous step); the lanes are joined again before the backtrack.
+    constexpr int G = jssp_server::kLanes;
+    JSSP_HIP(e, hipEventRecord(s->lane_ev[0], e->stream));
+    for (int g = 1; g < G; ++g)
+      if (s->mf_cnt[g] > 0) JSSP_HIP(e, hipStreamWaitEvent(s->lane_stream[g], s->lane_ev[0], 0));
     for (int t = 0; t < levels; ++t) {
-      if (s->nms > 0) JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms, s->nms, false, e->stream));
-      JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf, s->nmf, s->flds, s->filter_tp, e->stream));
-      for (int v = 0; v < 2 * jssp::kRankVariants; ++v)
-        if (s->mr_cnt[v] > 0)
-          JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[v], s->mr_cnt[v], s->rlds,
-                                             v % jssp::kRankVariants, jssp::kRankWavesThroughput, e->stream,
-                                             nullptr, v >= jssp::kRankVariants ? jssp::kRankBatch : 1));
+      for (int g = 0; g < G; ++g) {
+        if (s->mf_cnt[g] == 0) continue;
+        hipStream_t st = s->lane_stream[g];
+        if (s->ms_cnt[g] > 0)
+          JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms + s->ms_off[g], s->ms_cnt[g], false, st));
+        JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf + s->mf_off[g], s->mf_cnt[g], s->flds,
+                                             s->filter_tp, st));
+        for (int v = 0; v < 2 * jssp::kRankVariants; ++v)
+          if (s->mr_cnt[g][v] > 0)
+            JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[g][v], s->mr_cnt[g][v], s->rlds,
+                                               v % jssp::kRankVariants, jssp::kRankWavesThroughput, st, nullptr,
+                                               v >= jssp::kRankVariants ? jssp::kRankBatch : 1));
+      }
+    }
+    for (int g = 1; g < G; ++g) {
+      if (s->mf_cnt[g] == 0) continue;
+      JSSP_HIP(e, hipEventRecord(s->lane_ev[g], s->lane_stream[g]));
+      JSSP_HIP(e, hipStreamWaitEvent(e->stream, s->lane_ev[g], 0));
     }
     // This step's status, read by the next serve step (or by this one when
     // it runs no levels).
