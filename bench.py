@@ -22,17 +22,21 @@ Also reported on the same JSON line:
                    one-instance-at-a-time loop (same results checked)
   small_configs    10x5 K=64 BS-ICH (ICH_EXACT) over 32 instances and 20x20
                    K=1024 single-phase PMS_GPU / ICH_DET over 16 instances
-  serving          Poisson arrivals into the continuous-batching server vs the
-                   plain loop on the same request list: solves/s, p50/p99 at
-                   2x the plain loop's capacity and (load_sweep) at 0.5x, 1x
-                   and 4x; then every request queued at once (the server's
-                   capacity vs the plain loop's, 1 / mean service time)
+  serving          Poisson arrivals into the continuous-batching server and
+                   into the plain one-at-a-time loop, both run in real time
+                   on the same request list: solves/s, p50/p99 at 2x the
+                   plain loop's capacity and (load_sweep) at 0.5x, 1x and 4x;
+                   then every request queued at once (the server's capacity
+                   vs the plain loop's, 1 / mean service time)
   instance_seeds_1_8  the headline solve on the instances of generator seeds
                    1..8 (states/s over all eight, makespans)
   cpu_baseline     the reference's own BS-ICH (cpu/v0_ich) compiled into
                    oracle/_ref from the reference sources (or the oracle's
                    restatement when that library is absent), one thread, one
-                   solve of the same instance at K=2048 (~15 s)
+                   solve of the same instance at K=2048 (~15 s), beside the
+                   GPU's bit-exact ICH_EXACT solve of it; and (all_cores) the
+                   reference's BS-PMS (cpu/v1_pms) on every instance of the
+                   batch and serving workloads, one process per core
 """
 from __future__ import annotations
 
@@ -155,10 +159,28 @@ def pct(v, q):
     return v[min(len(v) - 1, max(0, int(round(q * (len(v) - 1)))))]
 
 
+def plain_realtime(eng, bs, reqs, arr, plain):
+    """The plain loop in real time on the same arrivals: one request at a
+    time, each solved as soon as it has arrived and the previous one is done.
+    Returns (latencies, span, results equal to the reference solves)."""
+    lat, t0 = [], time.perf_counter()
+    eq = True
+    for i, (inst, K) in enumerate(reqs):
+        wait = arr[i] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        r = eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345)
+        lat.append(time.perf_counter() - t0 - arr[i])
+        eq = eq and r.makespan == plain[i].makespan and (r.sequence == plain[i].sequence).all()
+    span = max(a + l for a, l in zip(arr, lat)) - arr[0]
+    return lat, span, eq
+
+
 def serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, load):
-    """One Poisson run at `load` x the plain loop's capacity: the plain loop's
-    latencies follow from the measured service times (a FIFO server on one
-    GPU), the server's are measured in real time on the same arrivals."""
+    """One Poisson run at `load` x the plain loop's capacity (1 / its mean
+    service time): the plain loop and the server each run in real time on
+    the same arrivals, and their latencies (completion - arrival) are
+    measured."""
     count = len(reqs)
     mean_service = sum(dur) / len(dur)
     rate = load / mean_service
@@ -166,11 +188,7 @@ def serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, load):
     for g in gaps:
         t += g / rate
         arr.append(t)
-    fin, lat_plain = 0.0, []
-    for a, d in zip(arr, dur):
-        fin = max(fin, a) + d
-        lat_plain.append(fin - a)
-    plain_span = fin - arr[0]
+    lat_plain, plain_span, eq_plain = plain_realtime(eng, bs, reqs, arr, plain)
     srv = bs.Server(eng, int(budget_gib * 2**30), max_slots=64)
     tickets, done, lat = [None] * count, {}, [0.0] * count
     nxt = 0
@@ -192,8 +210,8 @@ def serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, load):
             time.sleep(max(0.0, arr[nxt] - (time.perf_counter() - t0)))
     served_span = max(arr[i] + lat[i] for i in range(count)) - arr[0]
     srv.close()
-    eq = all(done[i].makespan == plain[i].makespan and (done[i].sequence == plain[i].sequence).all()
-             for i in range(count))
+    eq = eq_plain and all(done[i].makespan == plain[i].makespan and (done[i].sequence == plain[i].sequence).all()
+                          for i in range(count))
     return {"load": load, "arrival_rate_per_s": round(rate, 1),
             "solves_per_s": round(count / served_span, 2), "plain_solves_per_s": round(count / plain_span, 2),
             "p50_ms": round(pct(lat, 0.5) * 1e3, 2), "p99_ms": round(pct(lat, 0.99) * 1e3, 2),
@@ -238,6 +256,52 @@ def serving(eng, bs, count, budget_gib, load, sweep=(0.5, 1.0, 4.0)):
             "identical_results": eq}
 
 
+def _ref_pms_job(job):
+    """Worker of the all-cores baseline (a fresh process: no GPU state): the
+    reference's own BS-PMS, srand(seed); beam_search_pms(inst, K, 500000)."""
+    ref_so, n, m, mach, dur, K, seed = job
+    lib = ctypes.CDLL(ref_so)
+    I32P = ctypes.POINTER(ctypes.c_int32)
+    lib.ref_pms.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_int, ctypes.c_uint]
+    a = (ctypes.c_int32 * (n * m))(*mach)
+    b = (ctypes.c_int32 * (n * m))(*dur)
+    t0 = time.perf_counter()
+    ms = lib.ref_pms(n, m, a, b, K, 500000, seed)
+    return ms, time.perf_counter() - t0
+
+
+def cpu_all_cores(bs, insts, ks, cores):
+    """The reference CPU BS-PMS (cpu/v1_pms, the CPU form of the machine-focused
+    search the batch and the server run) on every instance of a workload,
+    one process per core, longest first: solves/s over the wall time."""
+    import multiprocessing
+    from concurrent.futures import ProcessPoolExecutor
+    ref_so = os.path.join(REPO, "oracle", "_ref", "libjssp_ref.so")
+    if not os.path.exists(ref_so):
+        return None
+    jobs = [(ref_so, i.nbjob, i.nbmachine, i.machine.ravel().tolist(), i.duration.ravel().tolist(), K, 12345)
+            for i, K in zip(insts, ks)]
+    order = sorted(range(len(jobs)), key=lambda j: -jobs[j][1] * jobs[j][1] * jobs[j][5])
+    # Workers start clean (spawn) and never touch the device; a worker that
+    # dies raises here instead of hanging the pool.
+    with ProcessPoolExecutor(cores, mp_context=multiprocessing.get_context("spawn")) as pool:
+        list(pool.map(_ref_pms_job, [jobs[order[-1]]] * cores))  # start every worker first
+        t0 = time.perf_counter()
+        res = list(pool.map(_ref_pms_job, [jobs[j] for j in order], chunksize=1))
+        wall = time.perf_counter() - t0
+    return {"solves_per_s": round(len(jobs) / wall, 3), "wall_s": round(wall, 2),
+            "cpu_s": round(sum(r[1] for r in res), 1), "longest_solve_s": round(max(r[1] for r in res), 2),
+            "solves": len(jobs)}
+
+
+def cpu_cores():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(inst, eng, bs, K=2048, seed=12345):
     """The reference BS-ICH on this host, one thread, same instance.  The
     number of states it expands is the GPU ICH_EXACT solve's count (the same
@@ -246,7 +310,9 @@ def cpu_baseline(inst, eng, bs, K=2048, seed=12345):
     n, m = inst.nbjob, inst.nbmachine
     I32P = ctypes.POINTER(ctypes.c_int32)
     mp, dp = inst.machine.ctypes.data_as(I32P), inst.duration.ctypes.data_as(I32P)
+    t0 = time.perf_counter()
     exact = eng.solve(inst, K, mode=bs.MODE_ICH_EXACT, seed=seed)
+    exact_wall = time.perf_counter() - t0
     if os.path.exists(ref_so):
         lib = ctypes.CDLL(ref_so)
         lib.ref_ich.argtypes = [ctypes.c_int, ctypes.c_int, I32P, I32P, ctypes.c_int, ctypes.c_uint]
@@ -266,8 +332,14 @@ def cpu_baseline(inst, eng, bs, K=2048, seed=12345):
     return {"value": round(exact.expanded_states / dt, 1), "unit": "states/s", "cores": 1, "kind": kind,
             "sample": f"one full solve of the same 50x20 instance at K={K} (srand({seed})), {label}; "
                       f"{exact.expanded_states} states in {dt:.2f} s; makespan {ms} == GPU ICH_EXACT "
-                      f"{exact.makespan}: {ms == exact.makespan}",
-            "cpu_model": _cpu_model()}
+                      f"{exact.makespan}: {ms == exact.makespan}. K={K} rather than the headline's 4096 "
+                      f"keeps the sample near 15 s (K=4096 takes about twice as long; the rate per state "
+                      f"changes little with K)",
+            "cpu_model": _cpu_model(),
+            "gpu_ich_exact": {"workload": f"the same solve on the GPU (ICH_EXACT: the reference's rand() "
+                                          f"stream and heap replayed, bit-exact), K={K}",
+                              "device_ms": round(exact.device_ms, 2), "wall_ms": round(exact_wall * 1e3, 2),
+                              "states_per_s": round(exact.expanded_states / (exact_wall), 1)}}
 
 
 def small_configs(eng, bs):
@@ -450,6 +522,17 @@ def main():
                 gpu_ms = line["small_configs"]["ich_exact_10x5_k64"]["makespans"]
                 small["equal_makespans"] = small["makespans"] == gpu_ms
             line["cpu_baseline"]["small_config"] = small
+        if not args.no_extras:
+            cores = cpu_cores()
+            log(f"CPU baseline: reference BS-PMS on {cores} cores, the batch and the serving workloads")
+            bi, bp = batch_workload(bs, args.batch)
+            allc = {"cores": cores, "kind": "reference",
+                    "solver": "cpu/v1_pms beam_search_pms compiled from the reference sources, srand(12345), "
+                              "best_known 500000, one process per core",
+                    "batch": cpu_all_cores(bs, bi, [p["beam_width"] for p in bp], cores)}
+            sreqs, _ = serve_workload(bs, args.requests)
+            allc["serving"] = cpu_all_cores(bs, [r[0] for r in sreqs], [r[1] for r in sreqs], cores)
+            line["cpu_baseline"]["all_cores"] = allc
     eng.close()
     print(json.dumps(line), flush=True)
 
