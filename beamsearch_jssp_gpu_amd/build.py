@@ -49,34 +49,45 @@ def _run(cmd: list[str]) -> None:
         sys.stderr.write(out + "\n")
 
 
-def build(verbose: bool = False) -> str:
-    """Compiles every source and links the shared library; returns its path."""
-    os.makedirs(OBJ, exist_ok=True)
-    os.makedirs(LIBDIR, exist_ok=True)
+# The phase-stamp build (beamsearch_jssp_gpu_amd/phase_stamps.py): the same
+# library with the kernels' wall-clock phase stamps compiled in, kept apart.
+STAMPS_OBJ = os.path.join(HERE, "_build_stamps")
+STAMPS_LIBDIR = os.path.join(HERE, "lib_stamps")
+STAMPS_LIB = os.path.join(STAMPS_LIBDIR, "libjssp_hip.so")
+
+
+def build(verbose: bool = False, stamps: bool = False) -> str:
+    """Compiles every source and links the shared library; returns its path.
+    stamps=True builds the phase-stamp variant into lib_stamps/ instead."""
+    obj_dir, lib_dir, lib = (STAMPS_OBJ, STAMPS_LIBDIR, STAMPS_LIB) if stamps else (OBJ, LIBDIR, LIB)
+    extra = ["-DJSSP_PHASE_STAMPS"] if stamps else []
+    os.makedirs(obj_dir, exist_ok=True)
+    os.makedirs(lib_dir, exist_ok=True)
     jobs = []
     objs = []
     for src in HIP_SOURCES:
         if not os.path.exists(os.path.join(CSRC, src)):
             continue
-        o = os.path.join(OBJ, src + ".o")
+        o = os.path.join(obj_dir, src + ".o")
         objs.append(o)
-        jobs.append([HIPCC, *HIP_FLAGS, "-x", "hip", "-c", "-o", o, os.path.join(CSRC, src)])
+        jobs.append([HIPCC, *HIP_FLAGS, *extra, "-x", "hip", "-c", "-o", o, os.path.join(CSRC, src)])
     for src in CXX_SOURCES:
         if not os.path.exists(os.path.join(CSRC, src)):
             continue
-        o = os.path.join(OBJ, src + ".o")
+        o = os.path.join(obj_dir, src + ".o")
         objs.append(o)
         jobs.append(["g++", *CXX_FLAGS, "-c", "-o", o, os.path.join(CSRC, src)])
     with ThreadPoolExecutor(max_workers=min(4, len(jobs))) as ex:
         list(ex.map(_run, jobs))
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
           f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath," + f"{ROCM}/lib"])
-    os.replace(tmp, LIB)
-    build_drop_in_driver()
+    os.replace(tmp, lib)
+    if not stamps:
+        build_drop_in_driver()
     if verbose:
-        print(LIB)
-    return LIB
+        print(lib)
+    return lib
 
 
 REPO = os.path.dirname(HERE)
@@ -100,4 +111,4 @@ def build_drop_in_driver() -> None:
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    build(verbose=True, stamps="--stamps" in sys.argv[1:])

@@ -58,6 +58,34 @@ __device__ __forceinline__ GL T* gp(T* p) {
   return (GL T*)p;
 }
 
+// Optional phase stamps, compiled in only with -DJSSP_PHASE_STAMPS (a
+// separate build, see beamsearch_jssp_gpu_amd/phase_stamps.py): wall-clock
+// (100 MHz) stamps at phase boundaries of the filter (thread 0 of each block)
+// and the single-solve rank kernel (lane 0 of each wave), for levels
+// [kStampLevel0, kStampLevel0 + kStampLevels) of a solve; read back with
+// jssp_debug_phase_stamps.
+#ifdef JSSP_PHASE_STAMPS
+constexpr int kStampLevel0 = 100, kStampLevels = 8, kStampRankBlocks = 256, kStampFilterBlocks = 512;
+__device__ unsigned long long g_rank_stamps[kStampLevels][kStampRankBlocks][16][8];
+__device__ unsigned long long g_filter_stamps[kStampLevels][kStampFilterBlocks][4];
+__device__ __forceinline__ bool stamp_level(int level) {
+  return level >= kStampLevel0 && level < kStampLevel0 + kStampLevels;
+}
+#define RANK_STAMP(k)                                                                                   \
+  do {                                                                                                  \
+    if (W == 16 && stamp_level(a.level) && lane == 0 && local < kStampRankBlocks)                       \
+      g_rank_stamps[a.level - kStampLevel0][local][wave][k] = (unsigned long long)wall_clock64();       \
+  } while (0)
+#define FILTER_STAMP(k)                                                                                 \
+  do {                                                                                                  \
+    if (stamp_level(a.level) && tid == 0 && local < kStampFilterBlocks)                                 \
+      g_filter_stamps[a.level - kStampLevel0][local][k] = (unsigned long long)wall_clock64();           \
+  } while (0)
+#else
+#define RANK_STAMP(k) ((void)0)
+#define FILTER_STAMP(k) ((void)0)
+#endif
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -1025,6 +1053,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ int s_last;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   SelState* sel = a.sel;
+  FILTER_STAMP(0);
   // The next level's histogram (read by the filter one level back) starts at
   // zero before its score pass, which follows this kernel.
   for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
@@ -1069,6 +1098,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   if (lane == 0) { s_e[wave] = ex; s_km[wave] = km; }
   if (tid == 0) s_T = kNB - 1;
   lds_barrier();
+  FILTER_STAMP(1);
   uint32_t wofs = 0, v = 0;
   for (int w = 0; w < 16; ++w) {
     wofs += (w < wave) ? s_w[w] : 0u;
@@ -1096,6 +1126,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   if (ex0 < Kp && ex0 + h0 >= Kp) s_T = 2 * tid;
   if (ex0 + h0 < Kp && ex0 + h0 + h1 >= Kp) s_T = 2 * tid + 1;
   lds_barrier();
+  FILTER_STAMP(2);
   const uint32_t T1 = s_T;
   const uint32_t cum_lt = s_ex[T1];
   const uint32_t t1 = (T1 + 1 < (uint32_t)kNB ? s_ex[T1 + 1] : v) - cum_lt;
@@ -1156,6 +1187,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     }
     lds_barrier();
   }
+  FILTER_STAMP(3);
   if (single) return;  // the rank kernel takes the ties tile by tile
 
   // The last block to finish sees every block's candidates and cuts the
@@ -1358,6 +1390,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   __shared__ uint32_t s_mh[kRankWaves], s_ml[kRankWaves], s_w[kRankWaves];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tid = threadIdx.x;
+  RANK_STAMP(0);
   if (FUSED)
     for (int i = tid; i < kNB; i += blockDim.x) s_h[i] = 0;
   if (local == 0 && tid == 0) {
@@ -1438,6 +1471,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     if (tid == (int)blockDim.x - 1) s_tp[tiles] = run;
   }
   lds_barrier();  // staged keys and tile prefix (LDS)
+  RANK_STAMP(1);
   const Layout Ly = make_layout(a.n, a.m);
   const int S = Ly.S;
   const int NW = (S + 63) >> 6;
@@ -1523,6 +1557,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
       if (!single) load_parent<NWM>(a, Ly, lane, p, pw);
     }
     const int q = (int)cnt;
+    RANK_STAMP(2);
     int cm;
     if (single) {
       GL const int* src = gp(a.cbuf) + (size_t)x * child_stride(S);
@@ -1534,6 +1569,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
       int nw[kMaxWPL];
       cm = (order == kOrderErase) ? build_child<kOrderErase, NWM>(a, Ly, lane, p, rec, q, pw, nw)
                                   : build_child<kOrderSwap, NWM>(a, Ly, lane, p, rec, q, pw, nw);
+      RANK_STAMP(3);
       if (FUSED) {
         // Score the survivor's children for the next level from its words.
         int* vw = s_view + wave * S;
@@ -1551,6 +1587,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
         emit_children<NHM>(dn, sln, q, true, 64, lane, ~0ull, lane, v, s_h, acc, slot, keep, nvalid);
       }
     }
+    RANK_STAMP(4);
     w_min = min(w_min, cm);
     w_max = max(w_max, hi);
     w_minh = min(w_minh, hi);
@@ -1570,7 +1607,9 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     }
     a.rstat[local] = rs;
   }
+  RANK_STAMP(5);
   if (FUSED) finish_score_block(acc, s_h, a.bstat, gp(a.hist_next), local, sln.last);
+  RANK_STAMP(6);
 }
 
 // ---------------------------------------------------------------------------
@@ -2053,3 +2092,16 @@ hipError_t launch_backtrack(const LevelArgs& a, hipStream_t s) {
 }
 
 }  // namespace jssp
+
+#ifdef JSSP_PHASE_STAMPS
+// Phase stamps of the last solve (this build only): rank[kStampLevels][256][16][8],
+// filter[kStampLevels][512][4], 100 MHz ticks, zero where not reached.
+extern "C" int jssp_debug_phase_stamps(void* rank_out, void* filter_out) {
+  if (hipMemcpyFromSymbol(rank_out, HIP_SYMBOL(jssp::g_rank_stamps), sizeof(jssp::g_rank_stamps)) != hipSuccess)
+    return 1;
+  if (hipMemcpyFromSymbol(filter_out, HIP_SYMBOL(jssp::g_filter_stamps), sizeof(jssp::g_filter_stamps)) !=
+      hipSuccess)
+    return 2;
+  return 0;
+}
+#endif

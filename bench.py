@@ -129,11 +129,11 @@ def batch_workload(bs, count=256, gen_seed=2026):
     return insts, params
 
 
-def batched(eng, bs, count):
+def batched(eng, bs, count, use_graph=True):
     insts, params = batch_workload(bs, count)
-    eng.solve_batch(insts, params)  # warm-up: autotunes the lane shapes, captures the step graph
+    eng.solve_batch(insts, params, use_graph=use_graph)  # warm-up: autotunes the lane shapes, captures the graph
     t0 = time.perf_counter()
-    rb = eng.solve_batch(insts, params)
+    rb = eng.solve_batch(insts, params, use_graph=use_graph)
     t_batch = time.perf_counter() - t0
     t0 = time.perf_counter()
     rp = [eng.solve(i, p["beam_width"], mode=p["mode"], seed=p["seed"]) for i, p in zip(insts, params)]
@@ -406,6 +406,9 @@ def main():
     ap.add_argument("--phases", type=int, default=2, choices=[1, 2])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extras", action="store_true", help="headline and roofline only")
+    ap.add_argument("--batch-only", action="store_true", help="the batched measurement only")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batches enqueue their step kernels directly instead of replaying captured graphs")
     ap.add_argument("--serving-only", action="store_true",
                     help="only the serving measurement (its JSON object), e.g. under a profiler")
     ap.add_argument("--cpu-beam", type=int, default=2048)
@@ -434,6 +437,12 @@ def main():
         sys.exit(2)
 
     import beamsearch_jssp_gpu_amd as bs
+    if args.batch_only:
+        eng = bs.Engine(0)
+        out = batched(eng, bs, args.batch, use_graph=not args.no_graph)
+        eng.close()
+        print(json.dumps({"batched": out, "use_graph": not args.no_graph}), flush=True)
+        return
     if args.serving_only:
         eng = bs.Engine(0)
         out = serving(eng, bs, args.requests, args.budget_gib, args.serve_load)
