@@ -116,6 +116,8 @@ _SIGNATURES = {
                                        ctypes.POINTER(_Result), _I32P]),
     "jssp_engine_set_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p]),
     "jssp_engine_tuned_lanes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "jssp_engine_tuned_shape": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               _I32P, _I32P, _I32P, _I32P]),
     "jssp_engine_set_profiling": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32]),
     "jssp_engine_kernel_time": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
@@ -370,6 +372,14 @@ class Engine:
 
     def tuned_lanes(self, n_jobs: int, n_machines: int, beam_width: int) -> int:
         return int(self.lib.jssp_engine_tuned_lanes(self.h, n_jobs, n_machines, beam_width))
+
+    def tuned_shape(self, n_jobs: int, n_machines: int, beam_width: int) -> Optional[dict]:
+        """The class's tuned batch shape (lanes, rank_waves, rank_batch,
+        filter_blocks), or None if not tuned yet."""
+        v = [ctypes.c_int32() for _ in range(4)]
+        if not self.lib.jssp_engine_tuned_shape(self.h, n_jobs, n_machines, beam_width, *[ctypes.byref(x) for x in v]):
+            return None
+        return dict(zip(("lanes", "rank_waves", "rank_batch", "filter_blocks"), (x.value for x in v)))
 
     def set_profiling(self, on: bool):
         self._check(self.lib.jssp_engine_set_profiling(self.h, 1 if on else 0))

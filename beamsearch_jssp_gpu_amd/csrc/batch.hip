@@ -48,14 +48,17 @@ constexpr int kStepsPerGraph = 16;
 constexpr size_t kGraphCacheMax = 64;
 
 // Steps [first, first + steps) of a batch and the blocks of the instances
-// still running during them, per kernel kind.
+// still running during them, per kernel kind; rank blocks per kernel variant
+// and rank block shape (index v + kRankVariants * shape).
+constexpr int kRankGroups = kRankVariants * kRankShapes;
 struct Segment {
   int first = 0, steps = 0;
   int nms = 0, nmf = 0;
   BlockRef *d_ms = nullptr, *d_mf = nullptr;
-  int nmr[kRankVariants] = {};
-  BlockRef* d_mr[kRankVariants] = {};
-  size_t flds = 0, rlds = 0;
+  int nmr[kRankGroups] = {};
+  BlockRef* d_mr[kRankGroups] = {};
+  size_t flds = 0;
+  size_t rlds[kRankShapes] = {};
 };
 
 struct BatchPlan {
@@ -91,9 +94,10 @@ int grow_batch_arena(jssp_engine* e, size_t bytes) {
 }
 
 // Lays out and uploads a batch: packed op tables, per-instance workspaces,
-// descriptors, block maps and a shared result region.
+// descriptors, block maps and a shared result region.  codes[i] (or null:
+// all 0) is instance i's tuned shape code (lanes, rank shape, filter cap).
 int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp_params* params,
-               const int* lanes, bool shared_bound, BatchPlan& bp) {
+               const int* codes, bool shared_bound, BatchPlan& bp) {
   bp = BatchPlan();
   bp.count = count;
   std::vector<jssp_params> ps(params, params + count);
@@ -103,7 +107,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
   std::vector<int> ends;
   for (int i = 0; i < count; ++i) {
     ps[i].phases = 2;  // batches materialise survivors only; beams are identical either way
-    Shape s = make_shape(&insts[i], &ps[i], lanes ? lanes[i] : 0, kRankWavesThroughput, kRankBatch);
+    Shape s = make_batch_shape(&insts[i], &ps[i], codes ? codes[i] : 0);
     bp.shapes.push_back(s);
     Work w;
     work_need[i] = carve(s, nullptr, w);
@@ -125,13 +129,14 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     Segment sg;
     sg.first = j ? ends[j - 1] : 0;
     sg.steps = ends[j] - sg.first;
-    std::vector<BlockRef> ms, mf, mr[kRankVariants];
+    std::vector<BlockRef> ms, mf, mr[kRankGroups];
     for (int i = 0; i < count; ++i) {
       const Shape& s = bp.shapes[i];
       if (s.L <= sg.first) continue;  // finished before this segment
-      append_maps(s, i, ms, mf, mr[rank_variant(s, &ps[i])]);
+      const int rs = rank_shape_of(s);
+      append_maps(s, i, ms, mf, mr[rank_variant(s, &ps[i]) + kRankVariants * rs]);
       sg.flds = std::max(sg.flds, filter_lds_bytes(s, &ps[i]));
-      sg.rlds = std::max(sg.rlds, rank_lds_bytes(s));
+      sg.rlds[rs] = std::max(sg.rlds[rs], rank_lds_bytes(s));
     }
     std::vector<size_t> o;
     auto add = [&](const std::vector<BlockRef>& v) {
@@ -140,10 +145,10 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     };
     add(ms);
     add(mf);
-    for (int v = 0; v < kRankVariants; ++v) add(mr[v]);
+    for (int v = 0; v < kRankGroups; ++v) add(mr[v]);
     sg.nms = (int)ms.size();
     sg.nmf = (int)mf.size();
-    for (int v = 0; v < kRankVariants; ++v) sg.nmr[v] = (int)mr[v].size();
+    for (int v = 0; v < kRankGroups; ++v) sg.nmr[v] = (int)mr[v].size();
     offs.push_back(o);
     bp.segs.push_back(sg);
   }
@@ -169,7 +174,7 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
     Segment& sg = bp.segs[j];
     sg.d_ms = d_maps + offs[j][0];
     sg.d_mf = d_maps + offs[j][1];
-    for (int v = 0; v < kRankVariants; ++v) sg.d_mr[v] = d_maps + offs[j][2 + v];
+    for (int v = 0; v < kRankGroups; ++v) sg.d_mr[v] = d_maps + offs[j][2 + v];
   }
   bp.d_res = b.take<int>((size_t)count * 4);
   bp.d_exp = b.take<unsigned long long>(count);
@@ -227,10 +232,12 @@ int plan_batch(jssp_engine* e, int count, const jssp_instance* insts, const jssp
 int enqueue_step(jssp_engine* e, const BatchPlan& bp, const Segment& sg) {
   if (sg.nms > 0) JSSP_HIP(e, launch_step_score(bp.d_desc, sg.d_ms, sg.nms, false, e->stream));
   JSSP_HIP(e, launch_step_filter(bp.d_desc, sg.d_mf, sg.nmf, sg.flds, true, e->stream));
-  for (int v = 0; v < kRankVariants; ++v)
-    if (sg.nmr[v] > 0)
-      JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[v], sg.nmr[v], sg.rlds, v, kRankWavesThroughput, e->stream,
-                                    nullptr, kRankBatch));
+  for (int g = 0; g < kRankGroups; ++g) {
+    if (sg.nmr[g] == 0) continue;
+    const int v = g % kRankVariants, rs = g / kRankVariants;
+    JSSP_HIP(e, launch_step_rank(bp.d_desc, sg.d_mr[g], sg.nmr[g], sg.rlds[rs], v, rank_shape_waves(rs), e->stream,
+                                  nullptr, rank_shape_batch(rs)));
+  }
   return JSSP_OK;
 }
 
@@ -241,9 +248,9 @@ int enqueue_segment(jssp_engine* e, const BatchPlan& bp, const Segment& sg, int 
   int done = 0;
   if (use_graph && n >= kStepsPerGraph) {
     std::vector<uint64_t> key = {(uint64_t)(uintptr_t)bp.d_desc, (uint64_t)sg.nms, (uint64_t)sg.nmf,
-                                 (uint64_t)(uintptr_t)sg.d_ms, (uint64_t)(uintptr_t)sg.d_mf, (uint64_t)sg.flds,
-                                 (uint64_t)sg.rlds};
-    for (int v = 0; v < kRankVariants; ++v) {
+                                 (uint64_t)(uintptr_t)sg.d_ms, (uint64_t)(uintptr_t)sg.d_mf, (uint64_t)sg.flds};
+    for (int rs = 0; rs < kRankShapes; ++rs) key.push_back((uint64_t)sg.rlds[rs]);
+    for (int v = 0; v < kRankGroups; ++v) {
       key.push_back((uint64_t)sg.nmr[v]);
       key.push_back((uint64_t)(uintptr_t)sg.d_mr[v]);
     }
@@ -333,11 +340,17 @@ void load_tune_file(jssp_engine* e) {
   if (e->tune_path.empty()) return;
   std::ifstream f(e->tune_path);
   if (!f.good()) return;
-  // Format: one "n m kbits lanes" record per line.
-  int n, m, kb, g;
-  while (f >> n >> m >> kb >> g) {
+  // Format: one "n m kbits lanes [rank_shape [filter_cap_index]]" record per line.
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    int n, m, kb, g, rs = kRankShapeBatched, fc = 0;
+    if (!(in >> n >> m >> kb >> g)) continue;
+    in >> rs >> fc;
     const uint64_t key = ((uint64_t)n << 40) | ((uint64_t)m << 32) | (uint64_t)kb;
-    if (!lookup_tune(e, key) && (g == kLanesFused || g == 16 || g == 32 || g == 64)) e->tune.push_back({key, g});
+    if (lookup_tune(e, key) || !(g == kLanesFused || g == 16 || g == 32 || g == 64)) continue;
+    if (rs < 0 || rs >= kRankShapes || fc < 0 || fc > 3) continue;
+    e->tune.push_back({key, make_code(g, rs, fc)});
   }
 }
 
@@ -346,12 +359,46 @@ void save_tune_file(const jssp_engine* e) {
   std::ofstream f(e->tune_path);
   for (const auto& kv : e->tune)
     f << (kv.first >> 40) << " " << ((kv.first >> 32) & 0xFF) << " " << (kv.first & 0xFFFFFFFF) << " "
-      << kv.second << "\n";
+      << code_lanes(kv.second) << " " << code_rank_shape(kv.second) << " " << ((kv.second >> 12) & 0x3) << "\n";
 }
 
+// Device time of `steps` steps of a batch of `sub` with every member on
+// shape `code`, starting at step w0 (the steps before run untimed).
+int time_shape(jssp_engine* e, const std::vector<jssp_instance>& sub, const std::vector<jssp_params>& subp,
+               int code, int w0, int steps, double& ms_out) {
+  std::vector<int> codes(sub.size(), code);
+  BatchPlan bp;
+  int rc = plan_batch(e, (int)sub.size(), sub.data(), subp.data(), codes.data(), false, bp);
+  if (rc) return rc;
+  JSSP_HIP(e, launch_step_init(bp.d_desc, bp.d_inst, bp.count, e->stream));
+  rc = enqueue_steps(e, bp, 0, w0, false);
+  if (rc) return rc;
+  JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
+  rc = enqueue_steps(e, bp, w0, steps, false);
+  if (rc) return rc;
+  JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  float ms = 0.f;
+  JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
+  ms_out = ms;
+  return JSSP_OK;
+}
+
+// Children per level of the batch a class is timed on: the class's members
+// repeated until the batch fills the device the way a large mixed batch does
+// (a few members alone leave most CUs idle and would favour the shapes with
+// the most blocks, which lose once the device is full).
+constexpr long long kTuneChildren = 1ll << 22;
+constexpr int kTuneMaxMembers = 1024;
+
 // For each (n, m, K bucket) class of the batch not yet in the table, times a
-// few levels of a small batch of its instances with every valid lanes-per-
-// parent shape and keeps the fastest.  Results do not depend on the shape.
+// batch of its instances (repeated up to kTuneChildren children per level)
+// over up to 16 full-width levels (after the untimed first levels, which
+// grow the beam to full width), first with every valid lanes-per-parent
+// shape (the fused form, or a separate score kernel with 16 / 32 / 64 lanes),
+// then with the winner and each rank block shape (8-wave blocks ranking four
+// survivors per wave, or one; 16-wave blocks), then each filter block cap,
+// keeping the fastest at each stage.  Results do not depend on the shape.
 int autotune_classes(jssp_engine* e, int count, const jssp_instance* insts, const jssp_params* params) {
   if (!e->autotune) return JSSP_OK;
   if (e->tune.empty()) load_tune_file(e);
@@ -362,36 +409,52 @@ int autotune_classes(jssp_engine* e, int count, const jssp_instance* insts, cons
     const uint64_t key = tune_key(n, m, K);
     if (lookup_tune(e, key) || std::find(seen.begin(), seen.end(), key) != seen.end()) continue;
     seen.push_back(key);
-    std::vector<int> members;
-    for (int k = i; k < count && (int)members.size() < 8; ++k)
-      if (tune_key(insts[k].n_jobs, insts[k].n_machines, params[k].beam_width) == key) members.push_back(k);
     std::vector<jssp_instance> sub;
     std::vector<jssp_params> subp;
-    for (int k : members) { sub.push_back(insts[k]); subp.push_back(params[k]); }
-    const int steps = std::min(24, n * m);
-    int best_g = 64;
-    double best_t = 1e30;
+    for (int k = i; k < count && (int)sub.size() < 8; ++k)
+      if (tune_key(insts[k].n_jobs, insts[k].n_machines, params[k].beam_width) == key) {
+        sub.push_back(insts[k]);
+        subp.push_back(params[k]);
+      }
+    long long children = 0;
+    for (const auto& q : subp) children += (long long)q.beam_width * n;
+    for (size_t k = 0, s0 = sub.size(); children < kTuneChildren && (int)sub.size() < kTuneMaxMembers; ++k) {
+      sub.push_back(sub[k % s0]);
+      subp.push_back(subp[k % s0]);
+      children += (long long)subp.back().beam_width * n;
+    }
+    const int L = n * m;
+    // The beam reaches full width after about log_n(K) levels.
+    int grow = 1;
+    for (long long w = n; w < K && grow < L; w *= std::max(2, n)) ++grow;
+    const int steps = std::max(1, std::min(16, L / 2));
+    const int w0 = std::max(0, std::min(grow + 2, L - 1 - steps));
+    double t = 0.0, best_t = 1e30;
+    int best = make_code(64, kRankShapeBatched, 0);
     const bool can_fuse = fused_eligible(&insts[i], &params[i]);
     for (int g : {kLanesFused, 16, 32, 64}) {
       if (g == kLanesFused && !can_fuse) continue;
       if (g != kLanesFused && g < 64 && g < n) continue;
-      std::vector<int> lanes(sub.size(), g);
-      BatchPlan bp;
-      int rc = plan_batch(e, (int)sub.size(), sub.data(), subp.data(), lanes.data(), false, bp);
+      const int code = make_code(g, kRankShapeBatched, 0);
+      int rc = time_shape(e, sub, subp, code, w0, steps, t);
       if (rc) return rc;
-      JSSP_HIP(e, launch_step_init(bp.d_desc, bp.d_inst, bp.count, e->stream));
-      rc = enqueue_steps(e, bp, 0, 2, false);  // warm up
-      if (rc) return rc;
-      JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
-      rc = enqueue_steps(e, bp, 2, steps, false);
-      if (rc) return rc;
-      JSSP_HIP(e, hipEventRecord(e->ev_stop, e->stream));
-      JSSP_HIP(e, hipStreamSynchronize(e->stream));
-      float ms = 0.f;
-      JSSP_HIP(e, hipEventElapsedTime(&ms, e->ev_start, e->ev_stop));
-      if (ms < best_t) { best_t = ms; best_g = g; }
+      if (t < best_t) { best_t = t; best = code; }
     }
-    e->tune.push_back({key, best_g});
+    for (int rs : {kRankShapeThroughput, kRankShapeSingle}) {
+      const int code = make_code(code_lanes(best), rs, 0);
+      int rc = time_shape(e, sub, subp, code, w0, steps, t);
+      if (rc) return rc;
+      if (t < best_t) { best_t = t; best = code; }
+    }
+    const long long tiles = ((long long)K * n + kFilterThreads - 1) / kFilterThreads;
+    for (int fc : {1, 3}) {  // 16 or 256 filter blocks (the default is 64)
+      if (fc == 3 && tiles <= 64) continue;
+      const int code = make_code(code_lanes(best), code_rank_shape(best), fc);
+      int rc = time_shape(e, sub, subp, code, w0, steps, t);
+      if (rc) return rc;
+      if (t < best_t) { best_t = t; best = code; }
+    }
+    e->tune.push_back({key, best});
     added = true;
   }
   if (added) save_tune_file(e);
@@ -423,11 +486,11 @@ int solve_batch(jssp_engine* e, int count, const jssp_instance* insts, const jss
   JSSP_HIP(e, hipSetDevice(e->device));
   rc = autotune_classes(e, count, insts, params);
   if (rc) return rc;
-  std::vector<int> lanes(count);
+  std::vector<int> codes(count);
   for (int i = 0; i < count; ++i)
-    lanes[i] = lookup_tune(e, tune_key(insts[i].n_jobs, insts[i].n_machines, params[i].beam_width));
+    codes[i] = lookup_tune(e, tune_key(insts[i].n_jobs, insts[i].n_machines, params[i].beam_width));
   BatchPlan bp;
-  rc = plan_batch(e, count, insts, params, lanes.data(), false, bp);
+  rc = plan_batch(e, count, insts, params, codes.data(), false, bp);
   if (rc) return rc;
   JSSP_HIP(e, hipEventRecord(e->ev_start, e->stream));
   rc = run_plan(e, bp, use_graph);
@@ -558,7 +621,20 @@ int jssp_engine_set_autotune(jssp_engine* e, int32_t on, const char* table_path)
 
 int jssp_engine_tuned_lanes(const jssp_engine* e, int32_t n_jobs, int32_t n_machines, int32_t beam_width) {
   if (!e) return 0;
-  return jssp::lookup_tune(e, jssp::tune_key(n_jobs, n_machines, beam_width));
+  return jssp::code_lanes(jssp::lookup_tune(e, jssp::tune_key(n_jobs, n_machines, beam_width)));
+}
+
+int jssp_engine_tuned_shape(const jssp_engine* e, int32_t n_jobs, int32_t n_machines, int32_t beam_width,
+                            int32_t* lanes, int32_t* rank_waves, int32_t* rank_batch, int32_t* filter_blocks) {
+  if (!e) return 0;
+  const int code = jssp::lookup_tune(e, jssp::tune_key(n_jobs, n_machines, beam_width));
+  if (!code) return 0;
+  const int rs = jssp::code_rank_shape(code);
+  if (lanes) *lanes = jssp::code_lanes(code);
+  if (rank_waves) *rank_waves = jssp::rank_shape_waves(rs);
+  if (rank_batch) *rank_batch = jssp::rank_shape_batch(rs);
+  if (filter_blocks) *filter_blocks = jssp::code_filter_cap(code) ? jssp::code_filter_cap(code) : 64;
+  return 1;
 }
 
 }  // extern "C"

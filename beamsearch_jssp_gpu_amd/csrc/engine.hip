@@ -138,7 +138,8 @@ bool fused_eligible(const jssp_instance* in, const jssp_params* p) {
          exact_digit_applies(in);
 }
 
-Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves, int rank_batch) {
+Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves, int rank_batch,
+                 int filter_cap) {
   Shape s;
   s.n = in->n_jobs; s.m = in->n_machines;
   s.S = 2 * s.n + s.m + 1;
@@ -158,8 +159,8 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int r
   // Throughput shapes (batches, the server): at most 64 filter blocks per
   // instance, each taking several 1024-child tiles, so fewer blocks re-read
   // the instance's histogram copies.
-  s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads,
-                                               rank_waves == kRankWavesThroughput ? 64ll : 512ll));
+  const long long cap = filter_cap > 0 ? filter_cap : (rank_waves == kRankWavesThroughput ? 64 : 512);
+  s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads, cap));
   s.rank_waves = rank_waves == kRankWavesThroughput ? kRankWavesThroughput : kRankWaves;
   s.rank_batch = s.rank_waves == kRankWavesThroughput ? std::max(1, std::min(rank_batch, kRankBatch)) : 1;
   const int per_blk = s.rank_waves * s.rank_batch;

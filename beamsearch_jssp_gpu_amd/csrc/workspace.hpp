@@ -67,8 +67,37 @@ bool fused_eligible(const jssp_instance* in, const jssp_params* p);
 // the server).
 // rank_batch: survivors per rank wave at full width (kRankBatch for batches,
 // whose instances fill the chip together; 1 keeps every wave to one survivor).
+// filter_cap: most filter blocks per instance in the throughput shapes (0 =
+// the default, 64); single solves take up to 512.
 Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int rank_waves = kRankWaves,
-                 int rank_batch = 1);
+                 int rank_batch = 1, int filter_cap = 0);
+
+// A batch shape chosen per (n, m, log2 K) class by the autotuner, packed in
+// one int: lanes per parent (bits 0-7: kLanesFused, 16, 32 or 64; 0 =
+// automatic), the rank block shape (bits 8-11, RankShape) and the filter
+// block cap (bits 12-15, an index into kFilterCaps; 0 = the default).
+enum RankShape : int {
+  kRankShapeBatched = 0,     // 8-wave blocks, kRankBatch survivors per wave (the default)
+  kRankShapeThroughput = 1,  // 8-wave blocks, one survivor per wave
+  kRankShapeSingle = 2,      // 16-wave blocks, one survivor per wave
+  kRankShapes = 3
+};
+constexpr int kFilterCaps[4] = {0, 16, 64, 256};
+inline int code_lanes(int code) { return code & 0xFF; }
+inline int code_rank_shape(int code) { return (code >> 8) & 0xF; }
+inline int code_filter_cap(int code) { return kFilterCaps[(code >> 12) & 0x3]; }
+inline int make_code(int lanes, int rank_shape, int cap_index) { return lanes | (rank_shape << 8) | (cap_index << 12); }
+inline int rank_shape_waves(int shape) { return shape == kRankShapeSingle ? kRankWaves : kRankWavesThroughput; }
+inline int rank_shape_batch(int shape) { return shape == kRankShapeBatched ? kRankBatch : 1; }
+// The batch shape of a tuning code.
+inline Shape make_batch_shape(const jssp_instance* in, const jssp_params* p, int code) {
+  const int rs = code_rank_shape(code);
+  return make_shape(in, p, code_lanes(code), rank_shape_waves(rs), rank_shape_batch(rs), code_filter_cap(code));
+}
+// Rank shape of a batch Shape.
+inline int rank_shape_of(const Shape& s) {
+  return s.rank_waves == kRankWaves ? kRankShapeSingle : (s.rank_batch > 1 ? kRankShapeBatched : kRankShapeThroughput);
+}
 // Bytes of the workspace (base == nullptr) or carve it at base.
 size_t carve(const Shape& s, char* base, Work& w);
 LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p);

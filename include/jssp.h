@@ -241,18 +241,29 @@ int jssp_solve_batch(jssp_engine* engine, int32_t count, const jssp_instance* in
 int jssp_solve_batch_ex(jssp_engine* engine, int32_t count, const jssp_instance* insts,
                         const jssp_params* params, jssp_result* results, int32_t use_graph);
 
-/* Score-shape autotuner: per (n, m, K) class, either a separate score kernel
- * with 16, 32 or 64 lanes per parent (tile = 16 waves), or (lanes = 1) the
- * fused form where the rank kernel scores the next level, one wave per
- * parent (two-phase device modes with the exact selection digit only).  When
- * on (the default), a batch first times a few levels of each class it has not
- * seen with every valid shape and keeps the fastest; table_path (or NULL)
- * persists the table as text lines "n m log2K lanes".  Without a tuned entry
- * a solve uses the fused form where it applies.  The shape never changes
- * results. */
+/* Batch-shape autotuner: per (n, m, K) class of a batch, (1) the score shape:
+ * a separate score kernel with 16, 32 or 64 lanes per parent (tile = 16
+ * waves), or (lanes = 1) the fused form where the rank kernel scores the next
+ * level, one wave per parent (two-phase device modes with the exact selection
+ * digit only); (2) the rank block shape: 8-wave blocks ranking 4 survivors
+ * per wave in one pass over the staged keys, 8-wave blocks ranking 1, or
+ * 16-wave blocks ranking 1; (3) the filter block cap per instance (16, 64 or
+ * 256).  When on (the default), a batch first times each class it has not
+ * seen over 24 full-width levels (a quarter of the way into the solve), stage
+ * by stage, and keeps the fastest; table_path (or NULL) persists the table as
+ * text lines "n m log2K lanes rank_shape filter_cap_index" (rank_shape 0, 1, 2
+ * as listed; filter_cap_index 0 = default 64, 1 = 16, 2 = 64, 3 = 256; lines
+ * with only the first four fields keep the defaults).  Without a tuned entry
+ * a batch uses the fused form where it applies, 8-wave blocks ranking 4
+ * survivors per wave and 64 filter blocks.  The shape never changes results. */
 int jssp_engine_set_autotune(jssp_engine* engine, int32_t on, const char* table_path);
-/* Tuned shape of a class (1 = fused, else lanes per parent), 0 if not tuned yet. */
+/* Tuned score shape of a class (1 = fused, else lanes per parent), 0 if not tuned yet. */
 int jssp_engine_tuned_lanes(const jssp_engine* engine, int32_t n_jobs, int32_t n_machines, int32_t beam_width);
+/* Whole tuned shape of a class: returns 1 and fills the non-NULL outputs
+ * (lanes as jssp_engine_tuned_lanes; rank waves per block, survivors ranked
+ * per wave in one pass, filter blocks per instance), 0 if not tuned yet. */
+int jssp_engine_tuned_shape(const jssp_engine* engine, int32_t n_jobs, int32_t n_machines, int32_t beam_width,
+                            int32_t* lanes, int32_t* rank_waves, int32_t* rank_batch, int32_t* filter_blocks);
 
 /* Per-kernel timing accumulated over the solves run since profiling was
  * switched on (jssp_engine_set_profiling(engine, 1) also resets it), for

@@ -74,6 +74,57 @@ def test_batch_lane_shapes_do_not_change_results(engine, tmp_path):
         eng.close()
 
 
+def test_batch_rank_shapes_and_filter_caps_do_not_change_results(engine, tmp_path):
+    """Each rank block shape (8-wave blocks ranking four survivors per wave or
+    one, 16-wave blocks) and filter block cap (16, 64, 256) the autotuner can
+    choose gives the same beams, with the fused and a separate score shape."""
+    jobs = _mixed(12, 10, ns=(10, 20, 30, 50), ks=(64, 256, 1024))
+    want = [engine.solve(_b(i), **p) for i, p in jobs]
+    for lanes in (bs.LANES_FUSED, 64):
+        for rs, waves, nb in ((0, 8, 4), (1, 8, 1), (2, 16, 1)):
+            for fc, blocks in ((1, 16), (3, 256)):
+                table = tmp_path / f"tune_{lanes}_{rs}_{fc}.txt"
+                with open(table, "w") as f:
+                    for inst, p in jobs:
+                        kb = int(np.ceil(np.log2(p["beam_width"])))
+                        f.write(f"{inst.n} {inst.m} {kb} {_shape(lanes, inst.n)} {rs} {fc}\n")
+                eng = bs.Engine(0)
+                eng.set_autotune(True, str(table))
+                print("shape", lanes, rs, fc, flush=True)
+                got = eng.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+                for g, w in zip(got, want):
+                    _same(g, w)
+                inst, p = jobs[0]
+                sh = eng.tuned_shape(inst.n, inst.m, p["beam_width"])
+                assert sh == dict(lanes=_shape(lanes, inst.n), rank_waves=waves, rank_batch=nb, filter_blocks=blocks)
+                eng.close()
+
+
+def test_autotuner_records_a_whole_shape(tmp_path):
+    """A batch of classes not yet seen is tuned: every class gets a score
+    shape, a rank block shape and a filter cap, written to the table file and
+    read back by a new engine; the results equal the single solves."""
+    jobs = _mixed(13, 6, ns=(15, 30), ms=(5, 10), ks=(256,))
+    table = tmp_path / "tuned.txt"
+    eng = bs.Engine(0)
+    eng.set_autotune(True, str(table))
+    got = eng.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+    for (inst, p), g in zip(jobs, got):
+        _same(g, eng.solve(_b(inst), **p))
+        sh = eng.tuned_shape(inst.n, inst.m, p["beam_width"])
+        assert sh is not None and sh["rank_waves"] in (8, 16) and sh["rank_batch"] in (1, 4)
+        assert sh["filter_blocks"] in (16, 64, 256)
+    eng.close()
+    lines = [l.split() for l in open(table).read().splitlines()]
+    assert lines and all(len(l) == 6 for l in lines)
+    eng2 = bs.Engine(0)
+    eng2.set_autotune(True, str(table))
+    inst, p = jobs[0]
+    eng2.solve_batch([_b(inst)], [p])  # loads the table, tunes nothing new
+    assert eng2.tuned_shape(inst.n, inst.m, p["beam_width"]) is not None
+    eng2.close()
+
+
 def test_batch_repeated_runs_identical(engine):
     jobs = _mixed(3, 12)
     a = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
