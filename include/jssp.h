@@ -249,8 +249,9 @@ int jssp_solve_batch_ex(jssp_engine* engine, int32_t count, const jssp_instance*
  * per wave in one pass over the staged keys, 8-wave blocks ranking 1, or
  * 16-wave blocks ranking 1; (3) the filter block cap per instance (16, 64 or
  * 256).  When on (the default), a batch first times each class it has not
- * seen over 24 full-width levels (a quarter of the way into the solve), stage
- * by stage, and keeps the fastest; table_path (or NULL) persists the table as
+ * seen, stage by stage, on a batch of the class's instances repeated until
+ * about 4 M children per level fill the device, over up to 16 full-width
+ * levels, and keeps the fastest; table_path (or NULL) persists the table as
  * text lines "n m log2K lanes rank_shape filter_cap_index" (rank_shape 0, 1, 2
  * as listed; filter_cap_index 0 = default 64, 1 = 16, 2 = 64, 3 = 256; lines
  * with only the first four fields keep the defaults).  Without a tuned entry
