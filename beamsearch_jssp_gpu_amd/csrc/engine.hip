@@ -163,8 +163,7 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int r
   s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads, cap));
   s.rank_waves = rank_waves == kRankWavesThroughput ? kRankWavesThroughput : kRankWaves;
   s.rank_batch = s.rank_waves == kRankWavesThroughput ? std::max(1, std::min(rank_batch, kRankBatch)) : 1;
-  const int per_blk = s.rank_waves * s.rank_batch;
-  s.nblk_rank = (int)std::max(1, std::min((s.K + per_blk - 1) / per_blk, 16 * 256 / s.rank_waves));
+  s.nblk_rank = rank_blocks(s.K, s.rank_waves, s.rank_batch);
   return s;
 }
 

@@ -1964,6 +1964,69 @@ __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SMALL ? 8 : 6)
   rank_body<true, MODE, SMALL, W, NB>(a, crec_next, r.y, nblk);
 }
 
+// Folds a served request's per-block partials into entry 0 and clears the
+// rest (launch_fold_partials).  Every reader of the partials combines them
+// by sums and minima / maxima, so entry 0 alone carries the level's values.
+__global__ __launch_bounds__(256) void k_fold_partials(InstDesc* desc, const int4* jobs) {
+  __shared__ uint32_t s_v[4], s_e[4], s_rmh[4], s_rlh[4];
+  __shared__ int s_mc[4], s_rmc[4];
+  __shared__ u64 s_b[4];
+  const int4 job = jobs[blockIdx.x];
+  const LevelArgs& a = desc[job.x].base;
+  const int old = job.y, cap = job.z;
+  const bool fused = job.w != 0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t v = 0, ex = 0, mxh = 0, mnh = 0xFFFFFFFFu;
+  int mc = INT_MAX, rmc = INT_MAX;
+  u64 best = ~0ull;
+  for (int i = threadIdx.x; i < old; i += blockDim.x) {
+    if (fused) {
+      const BlockStat b = a.bstat[i];
+      v += b.valid;
+      ex += b.expanded;
+      mc = min(mc, b.min_cmax);
+      best = b.best < best ? b.best : best;
+    }
+    const RankStat r = a.rstat[i];
+    rmc = min(rmc, r.min_cmax);
+    mxh = max(mxh, r.max_hi);
+    mnh = min(mnh, r.min_hi);
+  }
+  v = wave_sum(v);
+  ex = wave_sum(ex);
+  mc = wave_min_i(mc);
+  best = wave_min64(best);
+  rmc = wave_min_i(rmc);
+  mxh = wave_max(mxh);
+  mnh = wave_min_u(mnh);
+  if (lane == 0) {
+    s_v[wave] = v; s_e[wave] = ex; s_mc[wave] = mc; s_b[wave] = best;
+    s_rmc[wave] = rmc; s_rmh[wave] = mxh; s_rlh[wave] = mnh;
+  }
+  __syncthreads();  // every entry is read before any is written
+  if (threadIdx.x == 0) {
+    BlockStat b = {0u, 0u, INT_MAX, 0u, ~0ull};
+    RankStat r = {INT_MAX, 0u, 0xFFFFFFFFu, 0u};
+    for (int w = 0; w < 4; ++w) {
+      b.valid += s_v[w];
+      b.expanded += s_e[w];
+      b.min_cmax = min(b.min_cmax, s_mc[w]);
+      b.best = s_b[w] < b.best ? s_b[w] : b.best;
+      r.min_cmax = min(r.min_cmax, s_rmc[w]);
+      r.max_hi = max(r.max_hi, s_rmh[w]);
+      r.min_hi = min(r.min_hi, s_rlh[w]);
+    }
+    if (fused) a.bstat[0] = b;
+    a.rstat[0] = r;
+  }
+  const BlockStat eb = {0u, 0u, INT_MAX, 0u, ~0ull};
+  const RankStat er = {INT_MAX, 0u, 0xFFFFFFFFu, 0u};
+  for (int i = 1 + (int)threadIdx.x; i < cap; i += blockDim.x) {
+    if (fused) a.bstat[i] = eb;
+    a.rstat[i] = er;
+  }
+}
+
 // Jump table of a single solve (base.jump): thread (g, q) follows survivor
 // q after the last level of 32-level group g (level L-2 for the last group)
 // down the trail to the group's first level and records its ancestor after
@@ -2064,6 +2127,12 @@ hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_
 
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s, int* status) {
   hipLaunchKernelGGL(k_backtrack_b, dim3(nblk), dim3(256), 0, s, desc, map, status);
+  return hipGetLastError();
+}
+
+hipError_t launch_fold_partials(InstDesc* desc, const int4* jobs, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fold_partials, dim3(count), dim3(256), 0, s, desc, jobs);
   return hipGetLastError();
 }
 

@@ -189,6 +189,12 @@ hipError_t launch_step_rank(InstDesc* desc, const BlockRef* map, int nblk, size_
 // whether it is finished after this launch.
 hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, hipStream_t s,
                                  int* status = nullptr);
+// Per-block partials of served requests whose rank block count changes
+// between levels: job (slot, old rank blocks, partial capacity, fused) folds
+// the request's rank partials (and, fused, its score partials, which the
+// rank blocks write) [0, old) into entry 0 and clears entries [1, capacity).
+// One block per job.
+hipError_t launch_fold_partials(InstDesc* desc, const int4* jobs, int count, hipStream_t s);
 // Single solve: fills base.jump (when set) for the backtrack; K, L of the solve.
 hipError_t launch_step_jump(InstDesc* desc, int K, int L, hipStream_t s);
 

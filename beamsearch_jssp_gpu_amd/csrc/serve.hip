@@ -59,6 +59,7 @@ struct jssp_server {
   char* meta = nullptr;           // descriptors + block maps
   InstDesc* d_desc = nullptr;
   BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_slots = nullptr;
+  int4* d_fold = nullptr;  // partial-fold jobs of requests whose rank block count changes
   size_t cap_map = 0;
   int nslots = 0;
   std::vector<uint64_t> slot_ticket;  // 0 = free
@@ -66,6 +67,7 @@ struct jssp_server {
   std::vector<Work> slot_work;
   std::vector<Region> slot_region;
   std::vector<int> slot_level;  // levels enqueued since the slot's request was admitted
+  std::vector<int> slot_rank_cap;  // rank-block partials the slot's workspace holds
   std::deque<uint64_t> queue;
   std::unordered_map<uint64_t, Request> reqs;
   uint64_t next_ticket = 1;
@@ -75,16 +77,18 @@ struct jssp_server {
   // advanced on its own stream (lane 0 is the engine's stream): requests are
   // independent, so one group's launch gaps and tails overlap the other's
   // kernels.  Group g's score and filter blocks are d_ms[ms_off[g], +
-  // ms_cnt[g]) and d_mf[mf_off[g], + mf_cnt[g]); its rank blocks of kernel
-  // variant v are d_mr[mr_off[g][v], + mr_cnt[g][v]), v + kRankVariants being
-  // the same variant's requests in the batched rank form.
+  // ms_cnt[g]) and d_mf[mf_off[g], + mf_cnt[g]); its rank blocks of group q
+  // are d_mr[mr_off[g][q], + mr_cnt[g][q]), where the group q of a request of
+  // the request's rank block shape rs (jssp::RankShape) and kernel variant v
+  // is entry rs * kRankVariants + v.
   static constexpr int kLanes = 2;
+  static constexpr int kRankGroups = jssp::kRankShapes * jssp::kRankVariants;
   hipStream_t lane_stream[kLanes] = {};
   hipEvent_t lane_ev[kLanes] = {};
   int ms_off[kLanes] = {}, ms_cnt[kLanes] = {}, mf_off[kLanes] = {}, mf_cnt[kLanes] = {};
-  int mr_off[kLanes][2 * jssp::kRankVariants] = {}, mr_cnt[kLanes][2 * jssp::kRankVariants] = {};
+  int mr_off[kLanes][kRankGroups] = {}, mr_cnt[kLanes][kRankGroups] = {};
   bool filter_tp = true;  // filter form of the current maps (see rebuild_maps)
-  size_t flds = 0, rlds = 0;
+  size_t flds = 0, rlds[jssp::kRankShapes] = {};
   std::vector<InstDesc> hdesc;
   // Pipelined steps: each step's last kernel writes every resident slot's
   // finished flag into pinned host memory (status[parity], no copy command)
@@ -109,9 +113,15 @@ namespace {
 
 // Filter blocks that fill the chip: one per CU.
 constexpr long long kFillBlocks = 256;
-// Requests waiting (the admitted one included) from which admission picks
-// the batched rank form.
-constexpr size_t kDeepQueue = 8;
+// Rank block shape of the resident requests, from their rank blocks in the
+// batched shape (8-wave blocks ranking kRankBatch survivors per wave): at
+// kRankFillBatched or more blocks the device is throughput-bound and takes
+// that shape; below it, every block counts toward the requests' latency and
+// they take 8-wave blocks ranking one survivor per wave, or, at most
+// kRankFillSingle blocks in the batched shape (a couple of requests), the
+// single-solve shape (16-wave blocks).
+constexpr long long kRankFillBatched = 512;
+constexpr long long kRankFillSingle = 256;
 
 bool arena_alloc(jssp_server* s, size_t bytes, Region& out) {
   bytes = (bytes + 255) & ~(size_t)255;
@@ -143,13 +153,15 @@ void arena_free(jssp_server* s, Region reg) {
   }
 }
 
-// Workspace bytes of a request (two-phase, tuned lanes per parent;
-// rank_batch survivors per rank wave).
-size_t footprint(jssp_server* s, const Request& r, Shape& shape, int rank_batch = 1) {
+// Workspace bytes of a request (two-phase, tuned lanes per parent).  The
+// shape has the most rank blocks of the three rank block shapes (8 waves,
+// one survivor per wave), so the workspace holds the per-block partials of
+// whichever shape the request runs in.
+size_t footprint(jssp_server* s, const Request& r, Shape& shape) {
   jssp_params q = r.params;
   q.phases = 2;
   const int lanes = jssp_engine_tuned_lanes(s->e, r.inst.n_jobs, r.inst.n_machines, q.beam_width);
-  shape = jssp::make_shape(&r.inst, &q, lanes, jssp::kRankWavesThroughput, rank_batch);
+  shape = jssp::make_shape(&r.inst, &q, lanes, jssp::kRankWavesThroughput, 1);
   Work w;
   return jssp::carve(shape, nullptr, w);
 }
@@ -168,12 +180,10 @@ int admit(jssp_server* s) {
     for (int i = 0; i < s->nslots; ++i)
       if (!s->slot_ticket[i]) { slot = i; break; }
     if (slot < 0) break;
-    // A request admitted with kDeepQueue or more waiting takes the batched
-    // rank form (a quarter of the rank blocks, four survivors per wave): the
-    // device is then throughput-bound; with few waiting, every block counts
-    // toward the request's latency.
+    // Admitted in the shape with the most rank blocks; rebuild_maps picks
+    // the shape it runs in.
     Shape shape;
-    const size_t bytes = footprint(s, r, shape, s->queue.size() >= kDeepQueue ? jssp::kRankBatch : 1);
+    const size_t bytes = footprint(s, r, shape);
     Region reg;
     if (!arena_alloc(s, bytes, reg)) break;
     Work w;
@@ -186,6 +196,7 @@ int admit(jssp_server* s) {
     s->hdesc[slot] = d;
     s->slot_ticket[slot] = s->queue.front();
     s->slot_shape[slot] = shape;
+    s->slot_rank_cap[slot] = shape.nblk_rank;
     s->slot_work[slot] = w;
     s->slot_region[slot] = reg;
     r.slot = slot;
@@ -231,9 +242,9 @@ int rebuild_maps(jssp_server* s) {
   jssp_engine* e = s->e;
   constexpr int G = jssp_server::kLanes;
   std::vector<BlockRef> ms, mf, mr, ma;
-  std::vector<BlockRef> msg[G], mfg[G], mrv[G][2 * jssp::kRankVariants];
+  std::vector<BlockRef> msg[G], mfg[G], mrv[G][jssp_server::kRankGroups];
   s->flds = 0;
-  s->rlds = 0;
+  for (auto& r : s->rlds) r = 0;
   // Filter form: the throughput one (at most 64 blocks per request, two
   // blocks per CU) while the resident requests' blocks fill the chip; with
   // few requests left, the single-solve one with up to 512 blocks per
@@ -243,6 +254,13 @@ int rebuild_maps(jssp_server* s) {
   for (int i = 0; i < s->nslots; ++i)
     if (s->slot_ticket[i]) tp_blocks += s->slot_shape[i].nblk_filter;
   s->filter_tp = tp_blocks >= kFillBlocks;
+  // Rank block shape (the same for every resident request, from their load).
+  long long rb_batched = 0;
+  for (int i = 0; i < s->nslots; ++i)
+    if (s->slot_ticket[i]) rb_batched += jssp::rank_blocks(s->slot_shape[i].K, jssp::kRankWavesThroughput, jssp::kRankBatch);
+  const int rshape = rb_batched >= kRankFillBatched ? jssp::kRankShapeBatched
+                     : rb_batched > kRankFillSingle ? jssp::kRankShapeThroughput
+                                                    : jssp::kRankShapeSingle;
   // Groups: largest requests first, each to the group with the least work
   // so far (work per level ~ survivors x snapshot words).
   std::vector<int> order;
@@ -259,9 +277,25 @@ int rebuild_maps(jssp_server* s) {
     load[g] += work(i);
   }
   std::vector<std::pair<int, int>> nbf_changed;  // (slot, filter blocks)
+  std::vector<int4> folds;                       // requests whose rank block count changes
   for (int i = 0; i < s->nslots; ++i) {
     if (!s->slot_ticket[i]) continue;
-    Shape sh = s->slot_shape[i];
+    Shape& cur = s->slot_shape[i];
+    if (jssp::rank_shape_of(cur) != rshape) {
+      // The rank blocks write per-block partials (and, fused, the score
+      // partials of the next level) that the next kernels read back over the
+      // block count: fold the old blocks' partials into the first entry before
+      // the count changes.
+      const int old = cur.nblk_rank;
+      cur.rank_waves = jssp::rank_shape_waves(rshape);
+      cur.rank_batch = jssp::rank_shape_batch(rshape);
+      cur.nblk_rank = jssp::rank_blocks(cur.K, cur.rank_waves, cur.rank_batch);
+      s->hdesc[i].nblk_rank = cur.nblk_rank;
+      s->hdesc[i].base.nblk_rank = cur.nblk_rank;
+      if (cur.fused) s->hdesc[i].base.nblk_score = cur.nblk_rank;
+      folds.push_back(make_int4(i, old, s->slot_rank_cap[i], cur.fused ? 1 : 0));
+    }
+    Shape sh = cur;
     if (!s->filter_tp) {
       const long long N = (long long)sh.K * sh.B;
       sh.nblk_filter = (int)std::max(1ll, std::min((N + jssp::kFilterThreads - 1) / jssp::kFilterThreads, 512ll));
@@ -272,11 +306,11 @@ int rebuild_maps(jssp_server* s) {
     }
     const int g = group[i];
     const int rv = jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params);
-    jssp::append_maps(sh, i, msg[g], mfg[g], mrv[g][rv + (sh.rank_batch > 1 ? jssp::kRankVariants : 0)]);
+    jssp::append_maps(sh, i, msg[g], mfg[g], mrv[g][rshape * jssp::kRankVariants + rv]);
     ma.push_back(make_int2(i, 0));
     const Request& r = s->reqs[s->slot_ticket[i]];
     s->flds = std::max(s->flds, jssp::filter_lds_bytes(sh, &r.params));
-    s->rlds = std::max(s->rlds, jssp::rank_lds_bytes(sh));
+    s->rlds[rshape] = std::max(s->rlds[rshape], jssp::rank_lds_bytes(sh));
   }
   for (int g = 0; g < G; ++g) {
     s->ms_off[g] = (int)ms.size();
@@ -285,7 +319,7 @@ int rebuild_maps(jssp_server* s) {
     s->mf_off[g] = (int)mf.size();
     s->mf_cnt[g] = (int)mfg[g].size();
     mf.insert(mf.end(), mfg[g].begin(), mfg[g].end());
-    for (int v = 0; v < 2 * jssp::kRankVariants; ++v) {
+    for (int v = 0; v < jssp_server::kRankGroups; ++v) {
       s->mr_off[g][v] = (int)mr.size();
       s->mr_cnt[g][v] = (int)mrv[g][v].size();
       mr.insert(mr.end(), mrv[g][v].begin(), mrv[g][v].end());
@@ -296,7 +330,7 @@ int rebuild_maps(jssp_server* s) {
   if (ms.size() > s->cap_map || mf.size() > s->cap_map || mr.size() > s->cap_map)
     return jssp::set_error(e, JSSP_ERR_LIMIT, "serve", "block map capacity exceeded");
   const size_t hbytes = (ms.size() + mf.size() + mr.size() + ma.size()) * sizeof(BlockRef) +
-                        nbf_changed.size() * sizeof(int);
+                        nbf_changed.size() * sizeof(int) + folds.size() * (sizeof(int4) + 3 * sizeof(int)) + 16;
   int rc = jssp::ensure_host_staging(e, hbytes + 64);
   if (rc) return rc;
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
@@ -316,6 +350,25 @@ int rebuild_maps(jssp_server* s) {
     JSSP_HIP(e, hipMemcpyAsync(&s->d_desc[c.first].nblk_filter, h + off, sizeof(int), hipMemcpyHostToDevice,
                                e->stream));
     off += sizeof(int);
+  }
+  if (!folds.empty()) {
+    // New rank block counts into the descriptors (only these fields: the
+    // device owns the rest), then the fold, before the next step's kernels.
+    for (const int4& f : folds) {
+      const InstDesc& hd = s->hdesc[f.x];
+      const int vals[3] = {hd.nblk_rank, hd.base.nblk_rank, hd.base.nblk_score};
+      int* dsts[3] = {&s->d_desc[f.x].nblk_rank, &s->d_desc[f.x].base.nblk_rank, &s->d_desc[f.x].base.nblk_score};
+      for (int k = 0; k < 3; ++k) {
+        std::memcpy(h + off, &vals[k], sizeof(int));
+        JSSP_HIP(e, hipMemcpyAsync(dsts[k], h + off, sizeof(int), hipMemcpyHostToDevice, e->stream));
+        off += sizeof(int);
+      }
+    }
+    off = (off + 15) & ~(size_t)15;
+    std::memcpy(h + off, folds.data(), folds.size() * sizeof(int4));
+    JSSP_HIP(e, hipMemcpyAsync(s->d_fold, h + off, folds.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
+    off += folds.size() * sizeof(int4);
+    JSSP_HIP(e, jssp::launch_fold_partials(s->d_desc, s->d_fold, (int)folds.size(), e->stream));
   }
   JSSP_HIP(e, hipStreamSynchronize(e->stream));
   s->maps_dirty = false;
@@ -379,6 +432,7 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   jssp::Bump b;
   b.take<InstDesc>(s->nslots);
   for (int k = 0; k < 4; ++k) b.take<BlockRef>(s->cap_map);
+  b.take<int4>(s->nslots);
   const size_t meta_bytes = b.off + 256;
   if (hipMalloc(&s->meta, meta_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -393,7 +447,9 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->d_mf = b.take<BlockRef>(s->cap_map);
   s->d_mr = b.take<BlockRef>(s->cap_map);
   s->d_slots = b.take<BlockRef>(s->cap_map);
+  s->d_fold = b.take<int4>(s->nslots);
   s->slot_ticket.assign(s->nslots, 0);
+  s->slot_rank_cap.assign(s->nslots, 0);
   s->slot_shape.resize(s->nslots);
   s->slot_work.resize(s->nslots);
   s->slot_region.resize(s->nslots);
@@ -512,11 +568,13 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
           JSSP_HIP(e, jssp::launch_step_score(s->d_desc, s->d_ms + s->ms_off[g], s->ms_cnt[g], false, st));
         JSSP_HIP(e, jssp::launch_step_filter(s->d_desc, s->d_mf + s->mf_off[g], s->mf_cnt[g], s->flds,
                                              s->filter_tp, st));
-        for (int v = 0; v < 2 * jssp::kRankVariants; ++v)
-          if (s->mr_cnt[g][v] > 0)
-            JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[g][v], s->mr_cnt[g][v], s->rlds,
-                                               v % jssp::kRankVariants, jssp::kRankWavesThroughput, st, nullptr,
-                                               v >= jssp::kRankVariants ? jssp::kRankBatch : 1));
+        for (int v = 0; v < jssp_server::kRankGroups; ++v) {
+          if (s->mr_cnt[g][v] == 0) continue;
+          const int rs = v / jssp::kRankVariants;
+          JSSP_HIP(e, jssp::launch_step_rank(s->d_desc, s->d_mr + s->mr_off[g][v], s->mr_cnt[g][v], s->rlds[rs],
+                                             v % jssp::kRankVariants, jssp::rank_shape_waves(rs), st, nullptr,
+                                             jssp::rank_shape_batch(rs)));
+        }
       }
     }
     for (int g = 1; g < G; ++g) {

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "engine.hpp"
@@ -93,6 +94,13 @@ inline int rank_shape_batch(int shape) { return shape == kRankShapeBatched ? kRa
 inline Shape make_batch_shape(const jssp_instance* in, const jssp_params* p, int code) {
   const int rs = code_rank_shape(code);
   return make_shape(in, p, code_lanes(code), rank_shape_waves(rs), rank_shape_batch(rs), code_filter_cap(code));
+}
+// Rank blocks of a solve of beam width K in the given block shape: one
+// survivor per wave and pass (batch survivors per wave in the batched
+// form), at most 4096 waves.
+inline int rank_blocks(int K, int waves, int batch) {
+  const int per_blk = waves * batch;
+  return std::max(1, std::min((K + per_blk - 1) / per_blk, 16 * 256 / waves));
 }
 // Rank shape of a batch Shape.
 inline int rank_shape_of(const Shape& s) {
