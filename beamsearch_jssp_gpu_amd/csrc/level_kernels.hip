@@ -2027,6 +2027,13 @@ __global__ __launch_bounds__(256) void k_fold_partials(InstDesc* desc, const int
   }
 }
 
+__global__ __launch_bounds__(256) void k_scatter(const uint32_t* block, const CopyJob* jobs) {
+  const CopyJob j = jobs[blockIdx.x];
+  uint32_t* dst = reinterpret_cast<uint32_t*>(j.dst);
+  const uint32_t* src = block + j.src;
+  for (uint32_t i = threadIdx.x; i < j.words; i += blockDim.x) dst[i] = src[i];
+}
+
 // Jump table of a single solve (base.jump): thread (g, q) follows survivor
 // q after the last level of 32-level group g (level L-2 for the last group)
 // down the trail to the group's first level and records its ancestor after
@@ -2133,6 +2140,12 @@ hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, 
 hipError_t launch_fold_partials(InstDesc* desc, const int4* jobs, int count, hipStream_t s) {
   if (count <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_fold_partials, dim3(count), dim3(256), 0, s, desc, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter(const uint32_t* block, const CopyJob* jobs, int count, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter, dim3(count), dim3(256), 0, s, block, jobs);
   return hipGetLastError();
 }
 

@@ -195,6 +195,15 @@ hipError_t launch_step_backtrack(InstDesc* desc, const BlockRef* map, int nblk, 
 // rank blocks write) [0, old) into entry 0 and clears entries [1, capacity).
 // One block per job.
 hipError_t launch_fold_partials(InstDesc* desc, const int4* jobs, int count, hipStream_t s);
+// Copies of 4-byte words from one uploaded block to scattered device
+// destinations (the server's op tables, descriptors and descriptor fields):
+// job i copies words [src, src + words) of the block to dst.  One block per job.
+struct CopyJob {
+  unsigned long long dst;  // device address, 4-byte aligned
+  uint32_t src;            // word offset into the uploaded block
+  uint32_t words;
+};
+hipError_t launch_scatter(const uint32_t* block, const CopyJob* jobs, int count, hipStream_t s);
 // Single solve: fills base.jump (when set) for the backtrack; K, L of the solve.
 hipError_t launch_step_jump(InstDesc* desc, int K, int L, hipStream_t s);
 

@@ -57,9 +57,30 @@ struct jssp_server {
   size_t arena_bytes = 0;
   std::vector<Region> free_list;  // sorted by offset, coalesced
   char* meta = nullptr;           // descriptors + block maps
+  BlockRef* d_maps = nullptr;     // the four block maps, contiguous (d_ms .. d_slots point into it)
   InstDesc* d_desc = nullptr;
   BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_slots = nullptr;
-  int4* d_fold = nullptr;  // partial-fold jobs of requests whose rank block count changes
+  // Uploads never wait for the device: the host writes each one into one of
+  // two pinned staging buffers, alternately (a buffer is reused once the
+  // event recorded after its last copy has passed, long before in practice),
+  // and the copies go on the engine's stream behind the step in flight, which
+  // they therefore cannot disturb.  Scattered destinations (op tables,
+  // descriptors, descriptor fields) are written by one copy of a packed block
+  // into d_up and a scatter kernel, instead of one small copy each.
+  struct Stage {
+    char* h = nullptr;
+    size_t cap = 0;
+    hipEvent_t ev = nullptr;
+    bool pending = false;
+  };
+  Stage stage[2];
+  int stage_cur = 0;
+  char* d_up = nullptr;  // device block of the current upload (grows)
+  size_t up_cap = 0;
+  // Pinned landing buffer of finished requests' results (grows): copies
+  // into pageable memory would wait for the device's other queues.
+  char* hv = nullptr;
+  size_t hv_cap = 0;
   size_t cap_map = 0;
   int nslots = 0;
   std::vector<uint64_t> slot_ticket;  // 0 = free
@@ -123,6 +144,7 @@ constexpr long long kFillBlocks = 256;
 constexpr long long kRankFillBatched = 512;
 constexpr long long kRankFillSingle = 256;
 
+
 bool arena_alloc(jssp_server* s, size_t bytes, Region& out) {
   bytes = (bytes + 255) & ~(size_t)255;
   for (size_t i = 0; i < s->free_list.size(); ++i) {
@@ -152,6 +174,62 @@ void arena_free(jssp_server* s, Region reg) {
     s->free_list.erase(it);
   }
 }
+
+// Pinned staging memory of at least `bytes` that no copy in flight reads
+// (the current buffer of the pair).
+int stage_take(jssp_server* s, size_t bytes, char** out) {
+  jssp_engine* e = s->e;
+  jssp_server::Stage& st = s->stage[s->stage_cur];
+  if (st.pending) {
+    JSSP_HIP(e, hipEventSynchronize(st.ev));
+    st.pending = false;
+  }
+  if (st.cap < bytes) {
+    const size_t cap = std::max({bytes, 2 * st.cap, (size_t)64 << 10});
+    if (st.h) JSSP_HIP(e, hipHostFree(st.h));
+    st.h = nullptr;
+    st.cap = 0;
+    if (hipHostMalloc((void**)&st.h, cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      st.h = nullptr;
+      return jssp::set_error(e, JSSP_ERR_NOMEM, "serve", "cannot allocate pinned staging memory");
+    }
+    st.cap = cap;
+  }
+  *out = st.h;
+  return JSSP_OK;
+}
+
+// Marks the current staging buffer as read by the copies just enqueued on
+// the engine's stream and switches to the other one.
+int stage_done(jssp_server* s) {
+  jssp_server::Stage& st = s->stage[s->stage_cur];
+  JSSP_HIP(s->e, hipEventRecord(st.ev, s->e->stream));
+  st.pending = true;
+  s->stage_cur ^= 1;
+  return JSSP_OK;
+}
+
+// Device upload block of at least `bytes`.  Growing it waits for the
+// enqueued work that may still read the old one.
+int ensure_up(jssp_server* s, size_t bytes) {
+  jssp_engine* e = s->e;
+  if (bytes <= s->up_cap) return JSSP_OK;
+  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  if (s->d_up) JSSP_HIP(e, hipFree(s->d_up));
+  s->d_up = nullptr;
+  const size_t cap = std::max({bytes, 2 * s->up_cap, (size_t)256 << 10});
+  s->up_cap = 0;
+  if (hipMalloc(&s->d_up, cap) != hipSuccess) {
+    (void)hipGetLastError();
+    s->d_up = nullptr;
+    return jssp::set_error(e, JSSP_ERR_NOMEM, "serve", "cannot allocate the upload block");
+  }
+  s->up_cap = cap;
+  return JSSP_OK;
+}
+
+inline size_t round16(size_t b) { return (b + 15) & ~(size_t)15; }
 
 // Workspace bytes of a request (two-phase, tuned lanes per parent).  The
 // shape has the most rank blocks of the three rank block shapes (8 waves,
@@ -206,34 +284,48 @@ int admit(jssp_server* s) {
     fresh.push_back(slot);
   }
   if (fresh.empty()) return JSSP_OK;
-  // Upload op tables and descriptors of the admitted requests, then init them.
-  size_t hbytes = fresh.size() * (sizeof(InstDesc) + sizeof(BlockRef));
-  for (int slot : fresh) hbytes += (size_t)s->slot_shape[slot].n * s->slot_shape[slot].m * sizeof(int2);
-  int rc = jssp::ensure_host_staging(e, hbytes + 256);
-  if (rc) return rc;
-  JSSP_HIP(e, hipStreamSynchronize(e->stream));
-  char* h = e->hbuf;
-  size_t off = 0;
+  // One upload of the admitted requests' op tables and descriptors, the
+  // init map and the copy jobs that put the tables and descriptors in place;
+  // then the init kernel.  All of it runs behind the step in flight.
+  std::vector<jssp::CopyJob> jobs;
   std::vector<BlockRef> init_map;
+  size_t off = 0;
   for (int slot : fresh) {
-    const Request& r = s->reqs[s->slot_ticket[slot]];
-    int2* hops = reinterpret_cast<int2*>(h + off);
-    const int tot = r.inst.n_jobs * r.inst.n_machines;
-    for (int q = 0; q < tot; ++q) { hops[q].x = r.inst.machine[q]; hops[q].y = r.inst.duration[q]; }
-    JSSP_HIP(e, hipMemcpyAsync(s->slot_work[slot].ops, hops, (size_t)tot * sizeof(int2), hipMemcpyHostToDevice,
-                               e->stream));
-    off += (size_t)tot * sizeof(int2);
-    off = (off + 15) & ~(size_t)15;
-    std::memcpy(h + off, &s->hdesc[slot], sizeof(InstDesc));
-    JSSP_HIP(e, hipMemcpyAsync(s->d_desc + slot, h + off, sizeof(InstDesc), hipMemcpyHostToDevice, e->stream));
-    off += sizeof(InstDesc);
+    const size_t tot = (size_t)s->slot_shape[slot].n * s->slot_shape[slot].m;
+    jobs.push_back({(unsigned long long)(uintptr_t)s->slot_work[slot].ops, (uint32_t)(off / 4), (uint32_t)(tot * 2)});
+    off = round16(off + tot * sizeof(int2));
+    jobs.push_back({(unsigned long long)(uintptr_t)(s->d_desc + slot), (uint32_t)(off / 4),
+                    (uint32_t)(sizeof(InstDesc) / 4)});
+    off = round16(off + sizeof(InstDesc));
     init_map.push_back(make_int2(slot, 0));
   }
-  std::memcpy(h + off, init_map.data(), init_map.size() * sizeof(BlockRef));
-  JSSP_HIP(e, hipMemcpyAsync(s->d_slots, h + off, init_map.size() * sizeof(BlockRef), hipMemcpyHostToDevice,
-                             e->stream));
-  JSSP_HIP(e, jssp::launch_step_init(s->d_desc, s->d_slots, (int)init_map.size(), e->stream));
-  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  const size_t map_off = off;
+  off = round16(off + init_map.size() * sizeof(BlockRef));
+  const size_t job_off = off;
+  off = round16(off + jobs.size() * sizeof(jssp::CopyJob));
+  int rc = ensure_up(s, off);
+  if (rc) return rc;
+  char* h = nullptr;
+  rc = stage_take(s, off, &h);
+  if (rc) return rc;
+  for (size_t k = 0; k < fresh.size(); ++k) {
+    const int slot = fresh[k];
+    const Request& r = s->reqs[s->slot_ticket[slot]];
+    int2* hops = reinterpret_cast<int2*>(h + 4 * (size_t)jobs[2 * k].src);
+    const int tot = r.inst.n_jobs * r.inst.n_machines;
+    for (int q = 0; q < tot; ++q) { hops[q].x = r.inst.machine[q]; hops[q].y = r.inst.duration[q]; }
+    std::memcpy(h + 4 * (size_t)jobs[2 * k + 1].src, &s->hdesc[slot], sizeof(InstDesc));
+  }
+  std::memcpy(h + map_off, init_map.data(), init_map.size() * sizeof(BlockRef));
+  std::memcpy(h + job_off, jobs.data(), jobs.size() * sizeof(jssp::CopyJob));
+  JSSP_HIP(e, hipMemcpyAsync(s->d_up, h, off, hipMemcpyHostToDevice, e->stream));
+  rc = stage_done(s);
+  if (rc) return rc;
+  JSSP_HIP(e, jssp::launch_scatter(reinterpret_cast<const uint32_t*>(s->d_up),
+                                   reinterpret_cast<const jssp::CopyJob*>(s->d_up + job_off), (int)jobs.size(),
+                                   e->stream));
+  JSSP_HIP(e, jssp::launch_step_init(s->d_desc, reinterpret_cast<const BlockRef*>(s->d_up + map_off),
+                                     (int)init_map.size(), e->stream));
   s->maps_dirty = true;
   return JSSP_OK;
 }
@@ -327,50 +419,63 @@ int rebuild_maps(jssp_server* s) {
   }
   s->nmr = (int)mr.size();
   s->nact = (int)ma.size();
-  if (ms.size() > s->cap_map || mf.size() > s->cap_map || mr.size() > s->cap_map)
+  const size_t nmaps = ms.size() + mf.size() + mr.size() + ma.size();
+  if (nmaps > 4 * s->cap_map)
     return jssp::set_error(e, JSSP_ERR_LIMIT, "serve", "block map capacity exceeded");
-  const size_t hbytes = (ms.size() + mf.size() + mr.size() + ma.size()) * sizeof(BlockRef) +
-                        nbf_changed.size() * sizeof(int) + folds.size() * (sizeof(int4) + 3 * sizeof(int)) + 16;
-  int rc = jssp::ensure_host_staging(e, hbytes + 64);
-  if (rc) return rc;
-  JSSP_HIP(e, hipStreamSynchronize(e->stream));
-  char* h = e->hbuf;
-  size_t off = 0;
-  auto put = [&](BlockRef* dst, const std::vector<BlockRef>& v) -> int {
-    if (v.empty()) return JSSP_OK;
-    std::memcpy(h + off, v.data(), v.size() * sizeof(BlockRef));
-    JSSP_HIP(e, hipMemcpyAsync(dst, h + off, v.size() * sizeof(BlockRef), hipMemcpyHostToDevice, e->stream));
-    off += v.size() * sizeof(BlockRef);
-    return JSSP_OK;
+  // Staging: the four maps (one copy into d_maps), then the upload block:
+  // the changed descriptor fields, the fold jobs and the copy jobs that put
+  // the fields in place.
+  std::vector<int32_t> vals;
+  std::vector<jssp::CopyJob> jobs;
+  auto field = [&](int* dst, int v) {
+    jobs.push_back({(unsigned long long)(uintptr_t)dst, (uint32_t)vals.size(), 1u});
+    vals.push_back(v);
   };
-  if ((rc = put(s->d_ms, ms)) || (rc = put(s->d_mf, mf)) || (rc = put(s->d_mr, mr)) || (rc = put(s->d_slots, ma)))
-    return rc;
-  for (const auto& c : nbf_changed) {
-    std::memcpy(h + off, &c.second, sizeof(int));
-    JSSP_HIP(e, hipMemcpyAsync(&s->d_desc[c.first].nblk_filter, h + off, sizeof(int), hipMemcpyHostToDevice,
-                               e->stream));
-    off += sizeof(int);
+  for (const auto& c : nbf_changed) field(&s->d_desc[c.first].nblk_filter, c.second);
+  for (const int4& f : folds) {
+    // New rank block counts (only these fields: the device owns the rest).
+    const InstDesc& hd = s->hdesc[f.x];
+    field(&s->d_desc[f.x].nblk_rank, hd.nblk_rank);
+    field(&s->d_desc[f.x].base.nblk_rank, hd.base.nblk_rank);
+    field(&s->d_desc[f.x].base.nblk_score, hd.base.nblk_score);
   }
-  if (!folds.empty()) {
-    // New rank block counts into the descriptors (only these fields: the
-    // device owns the rest), then the fold, before the next step's kernels.
-    for (const int4& f : folds) {
-      const InstDesc& hd = s->hdesc[f.x];
-      const int vals[3] = {hd.nblk_rank, hd.base.nblk_rank, hd.base.nblk_score};
-      int* dsts[3] = {&s->d_desc[f.x].nblk_rank, &s->d_desc[f.x].base.nblk_rank, &s->d_desc[f.x].base.nblk_score};
-      for (int k = 0; k < 3; ++k) {
-        std::memcpy(h + off, &vals[k], sizeof(int));
-        JSSP_HIP(e, hipMemcpyAsync(dsts[k], h + off, sizeof(int), hipMemcpyHostToDevice, e->stream));
-        off += sizeof(int);
-      }
-    }
-    off = (off + 15) & ~(size_t)15;
-    std::memcpy(h + off, folds.data(), folds.size() * sizeof(int4));
-    JSSP_HIP(e, hipMemcpyAsync(s->d_fold, h + off, folds.size() * sizeof(int4), hipMemcpyHostToDevice, e->stream));
-    off += folds.size() * sizeof(int4);
-    JSSP_HIP(e, jssp::launch_fold_partials(s->d_desc, s->d_fold, (int)folds.size(), e->stream));
+  const size_t maps_bytes = round16(nmaps * sizeof(BlockRef));
+  const size_t fold_off = round16(vals.size() * sizeof(int32_t));
+  const size_t job_off = round16(fold_off + folds.size() * sizeof(int4));
+  const size_t up_bytes = round16(job_off + jobs.size() * sizeof(jssp::CopyJob));
+  int rc = ensure_up(s, up_bytes);
+  if (rc) return rc;
+  char* h = nullptr;
+  rc = stage_take(s, maps_bytes + up_bytes, &h);
+  if (rc) return rc;
+  size_t off = 0;
+  BlockRef* const dm = s->d_maps;
+  auto put = [&](BlockRef*& dst, const std::vector<BlockRef>& v) {
+    dst = dm + off / sizeof(BlockRef);
+    if (!v.empty()) std::memcpy(h + off, v.data(), v.size() * sizeof(BlockRef));
+    off += v.size() * sizeof(BlockRef);
+  };
+  put(s->d_ms, ms);
+  put(s->d_mf, mf);
+  put(s->d_mr, mr);
+  put(s->d_slots, ma);
+  char* hu = h + maps_bytes;
+  if (!vals.empty()) std::memcpy(hu, vals.data(), vals.size() * sizeof(int32_t));
+  if (!folds.empty()) std::memcpy(hu + fold_off, folds.data(), folds.size() * sizeof(int4));
+  if (!jobs.empty()) std::memcpy(hu + job_off, jobs.data(), jobs.size() * sizeof(jssp::CopyJob));
+  if (nmaps) JSSP_HIP(e, hipMemcpyAsync(dm, h, nmaps * sizeof(BlockRef), hipMemcpyHostToDevice, e->stream));
+  if (!jobs.empty()) {
+    JSSP_HIP(e, hipMemcpyAsync(s->d_up, hu, up_bytes, hipMemcpyHostToDevice, e->stream));
+    JSSP_HIP(e, jssp::launch_scatter(reinterpret_cast<const uint32_t*>(s->d_up),
+                                     reinterpret_cast<const jssp::CopyJob*>(s->d_up + job_off), (int)jobs.size(),
+                                     e->stream));
+    // The rank partials of requests whose block count changes, folded
+    // before the next step's kernels read them.
+    JSSP_HIP(e, jssp::launch_fold_partials(s->d_desc, reinterpret_cast<const int4*>(s->d_up + fold_off),
+                                           (int)folds.size(), e->stream));
   }
-  JSSP_HIP(e, hipStreamSynchronize(e->stream));
+  rc = stage_done(s);
+  if (rc) return rc;
   s->maps_dirty = false;
   return JSSP_OK;
 }
@@ -383,19 +488,51 @@ int harvest(jssp_server* s, int par) {
   jssp_engine* e = s->e;
   JSSP_HIP(e, hipEventSynchronize(s->status_ev[par]));
   const volatile int* done = s->status[par];
+  std::vector<int> fin;
+  std::vector<size_t> at;
+  size_t need = 0;
   for (int i = 0; i < s->nslots; ++i) {
     const uint64_t t = s->slot_ticket[i];
     if (!t || t != s->status_ticket[par][i] || !done[i]) continue;
-    Request& r = s->reqs[t];
-    const Work& w = s->slot_work[i];
-    int res[4];
-    unsigned long long ex = 0;
-    r.seq.assign((size_t)s->slot_shape[i].L, 0);
-    JSSP_HIP(e, hipMemcpyAsync(res, w.result, 3 * sizeof(int), hipMemcpyDeviceToHost, s->copy_stream));
-    JSSP_HIP(e, hipMemcpyAsync(&ex, w.expanded, sizeof(ex), hipMemcpyDeviceToHost, s->copy_stream));
-    JSSP_HIP(e, hipMemcpyAsync(r.seq.data(), w.seq, r.seq.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+    fin.push_back(i);
+    at.push_back(need);
+    need += round16(32 + (size_t)s->slot_shape[i].L * sizeof(int32_t));
+  }
+  if (fin.empty()) return JSSP_OK;
+  if (s->hv_cap < need) {  // the buffer is idle here: every harvest waits for its copies
+    const size_t cap = std::max({need, 2 * s->hv_cap, (size_t)256 << 10});
+    if (s->hv) JSSP_HIP(e, hipHostFree(s->hv));
+    s->hv = nullptr;
+    s->hv_cap = 0;
+    if (hipHostMalloc((void**)&s->hv, cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      s->hv = nullptr;
+      return jssp::set_error(e, JSSP_ERR_NOMEM, "serve", "cannot allocate the result landing buffer");
+    }
+    s->hv_cap = cap;
+  }
+  // Each finished request: result (makespan, found, best) at +0, expanded
+  // count at +16, sequence at +32.
+  for (size_t k = 0; k < fin.size(); ++k) {
+    const Work& w = s->slot_work[fin[k]];
+    char* d = s->hv + at[k];
+    JSSP_HIP(e, hipMemcpyAsync(d, w.result, 3 * sizeof(int), hipMemcpyDeviceToHost, s->copy_stream));
+    JSSP_HIP(e, hipMemcpyAsync(d + 16, w.expanded, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                s->copy_stream));
-    JSSP_HIP(e, hipStreamSynchronize(s->copy_stream));
+    JSSP_HIP(e, hipMemcpyAsync(d + 32, w.seq, (size_t)s->slot_shape[fin[k]].L * sizeof(int32_t),
+                               hipMemcpyDeviceToHost, s->copy_stream));
+  }
+  JSSP_HIP(e, hipStreamSynchronize(s->copy_stream));
+  for (size_t k = 0; k < fin.size(); ++k) {
+    const int i = fin[k];
+    Request& r = s->reqs[s->slot_ticket[i]];
+    const char* d = s->hv + at[k];
+    int res[3];
+    unsigned long long ex = 0;
+    std::memcpy(res, d, sizeof(res));
+    std::memcpy(&ex, d + 16, sizeof(ex));
+    r.seq.assign(reinterpret_cast<const int32_t*>(d + 32),
+                 reinterpret_cast<const int32_t*>(d + 32) + s->slot_shape[i].L);
     r.makespan = res[0];
     r.found = res[1];
     r.expanded = (int64_t)ex;
@@ -431,8 +568,7 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->cap_map = (size_t)s->nslots * 512;
   jssp::Bump b;
   b.take<InstDesc>(s->nslots);
-  for (int k = 0; k < 4; ++k) b.take<BlockRef>(s->cap_map);
-  b.take<int4>(s->nslots);
+  b.take<BlockRef>(4 * s->cap_map);
   const size_t meta_bytes = b.off + 256;
   if (hipMalloc(&s->meta, meta_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -443,11 +579,8 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   b = jssp::Bump();
   b.base = s->meta;
   s->d_desc = b.take<InstDesc>(s->nslots);
-  s->d_ms = b.take<BlockRef>(s->cap_map);
-  s->d_mf = b.take<BlockRef>(s->cap_map);
-  s->d_mr = b.take<BlockRef>(s->cap_map);
-  s->d_slots = b.take<BlockRef>(s->cap_map);
-  s->d_fold = b.take<int4>(s->nslots);
+  s->d_maps = b.take<BlockRef>(4 * s->cap_map);
+  s->d_ms = s->d_mf = s->d_mr = s->d_slots = s->d_maps;
   s->slot_ticket.assign(s->nslots, 0);
   s->slot_rank_cap.assign(s->nslots, 0);
   s->slot_shape.resize(s->nslots);
@@ -455,6 +588,14 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->slot_region.resize(s->nslots);
   s->slot_level.assign(s->nslots, 0);
   s->hdesc.resize(s->nslots);
+  for (int k = 0; k < 2; ++k) {
+    if (hipEventCreateWithFlags(&s->stage[k].ev, hipEventDisableTiming) != hipSuccess) {
+      (void)hipGetLastError();
+      s->stage[k].ev = nullptr;
+      jssp_server_destroy(s);
+      return jssp::set_error(e, JSSP_ERR_NOMEM, "jssp_server_create", "cannot create the staging events");
+    }
+  }
   for (int k = 0; k < 2; ++k) {
     s->status_ticket[k].assign(s->nslots, 0);
     if (hipHostMalloc((void**)&s->status[k], (size_t)s->nslots * sizeof(int), hipHostMallocCoherent) != hipSuccess ||
@@ -500,10 +641,14 @@ void jssp_server_destroy(jssp_server* s) {
   }
   for (int k = 0; k < 2; ++k) {
     if (s->status_ev[k]) (void)hipEventDestroy(s->status_ev[k]);
+    if (s->stage[k].ev) (void)hipEventDestroy(s->stage[k].ev);
+    if (s->stage[k].h) (void)hipHostFree(s->stage[k].h);
     if (s->status[k]) (void)hipHostFree(s->status[k]);
   }
   if (s->arena) (void)hipFree(s->arena);
   if (s->meta) (void)hipFree(s->meta);
+  if (s->d_up) (void)hipFree(s->d_up);
+  if (s->hv) (void)hipHostFree(s->hv);
   delete s;
 }
 
