@@ -57,9 +57,9 @@ struct jssp_server {
   size_t arena_bytes = 0;
   std::vector<Region> free_list;  // sorted by offset, coalesced
   char* meta = nullptr;           // descriptors + block maps
-  BlockRef* d_maps = nullptr;     // the four block maps, contiguous (d_ms .. d_slots point into it)
+  BlockRef* d_maps = nullptr;     // the three block maps, contiguous (d_ms, d_mf, d_mr point into it)
   InstDesc* d_desc = nullptr;
-  BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr, *d_slots = nullptr;
+  BlockRef *d_ms = nullptr, *d_mf = nullptr, *d_mr = nullptr;
   // Uploads never wait for the device: the host writes each one into one of
   // two pinned staging buffers, alternately (a buffer is reused once the
   // event recorded after its last copy has passed, long before in practice),
@@ -111,16 +111,21 @@ struct jssp_server {
   bool filter_tp = true;  // filter form of the current maps (see rebuild_maps)
   size_t flds = 0, rlds[jssp::kRankShapes] = {};
   std::vector<InstDesc> hdesc;
-  // Pipelined steps: each step's last kernel writes every resident slot's
-  // finished flag into pinned host memory (status[parity], no copy command)
-  // and the step ends with an event; the next serve step reads that status
-  // after enqueuing its own kernels, so the device always has the current
-  // step queued while the host harvests.  status_ticket records which request
-  // held each slot when the step was enqueued (a slot freed and re-admitted
-  // since then is skipped).
+  // Pipelined steps: the backtrack of the requests that finish in a step
+  // writes their finished flags into pinned host memory (status[parity], no
+  // copy command) and the step ends with an event; the next serve step reads
+  // that status after enqueuing its own kernels, so the device always has
+  // the current step queued while the host harvests.  status_ticket records
+  // which request held each slot when the step was enqueued (a slot freed and
+  // re-admitted since then is skipped).
   int* status[2] = {nullptr, nullptr};
   int* status_dev[2] = {nullptr, nullptr};
   std::vector<uint64_t> status_ticket[2];
+  // The requests whose last level runs in the step of each parity, and
+  // their backtrack map (pinned host memory the backtrack kernel reads).
+  std::vector<int> status_fin[2];
+  BlockRef* fin_map[2] = {nullptr, nullptr};
+  BlockRef* fin_map_dev[2] = {nullptr, nullptr};
   hipEvent_t status_ev[2] = {nullptr, nullptr};
   int status_pending = -1;  // parity of the step whose status is not read yet
   int parity = 0;
@@ -333,7 +338,8 @@ int admit(jssp_server* s) {
 int rebuild_maps(jssp_server* s) {
   jssp_engine* e = s->e;
   constexpr int G = jssp_server::kLanes;
-  std::vector<BlockRef> ms, mf, mr, ma;
+  std::vector<BlockRef> ms, mf, mr;
+  int nlive = 0;
   std::vector<BlockRef> msg[G], mfg[G], mrv[G][jssp_server::kRankGroups];
   s->flds = 0;
   for (auto& r : s->rlds) r = 0;
@@ -399,7 +405,7 @@ int rebuild_maps(jssp_server* s) {
     const int g = group[i];
     const int rv = jssp::rank_variant(sh, &s->reqs[s->slot_ticket[i]].params);
     jssp::append_maps(sh, i, msg[g], mfg[g], mrv[g][rshape * jssp::kRankVariants + rv]);
-    ma.push_back(make_int2(i, 0));
+    ++nlive;
     const Request& r = s->reqs[s->slot_ticket[i]];
     s->flds = std::max(s->flds, jssp::filter_lds_bytes(sh, &r.params));
     s->rlds[rshape] = std::max(s->rlds[rshape], jssp::rank_lds_bytes(sh));
@@ -418,11 +424,11 @@ int rebuild_maps(jssp_server* s) {
     }
   }
   s->nmr = (int)mr.size();
-  s->nact = (int)ma.size();
-  const size_t nmaps = ms.size() + mf.size() + mr.size() + ma.size();
-  if (nmaps > 4 * s->cap_map)
+  s->nact = nlive;
+  const size_t nmaps = ms.size() + mf.size() + mr.size();
+  if (nmaps > 3 * s->cap_map)
     return jssp::set_error(e, JSSP_ERR_LIMIT, "serve", "block map capacity exceeded");
-  // Staging: the four maps (one copy into d_maps), then the upload block:
+  // Staging: the three maps (one copy into d_maps), then the upload block:
   // the changed descriptor fields, the fold jobs and the copy jobs that put
   // the fields in place.
   std::vector<int32_t> vals;
@@ -458,7 +464,6 @@ int rebuild_maps(jssp_server* s) {
   put(s->d_ms, ms);
   put(s->d_mf, mf);
   put(s->d_mr, mr);
-  put(s->d_slots, ma);
   char* hu = h + maps_bytes;
   if (!vals.empty()) std::memcpy(hu, vals.data(), vals.size() * sizeof(int32_t));
   if (!folds.empty()) std::memcpy(hu + fold_off, folds.data(), folds.size() * sizeof(int4));
@@ -491,7 +496,7 @@ int harvest(jssp_server* s, int par) {
   std::vector<int> fin;
   std::vector<size_t> at;
   size_t need = 0;
-  for (int i = 0; i < s->nslots; ++i) {
+  for (int i : s->status_fin[par]) {
     const uint64_t t = s->slot_ticket[i];
     if (!t || t != s->status_ticket[par][i] || !done[i]) continue;
     fin.push_back(i);
@@ -568,7 +573,7 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   s->cap_map = (size_t)s->nslots * 512;
   jssp::Bump b;
   b.take<InstDesc>(s->nslots);
-  b.take<BlockRef>(4 * s->cap_map);
+  b.take<BlockRef>(3 * s->cap_map);
   const size_t meta_bytes = b.off + 256;
   if (hipMalloc(&s->meta, meta_bytes) != hipSuccess) {
     (void)hipGetLastError();
@@ -579,8 +584,8 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
   b = jssp::Bump();
   b.base = s->meta;
   s->d_desc = b.take<InstDesc>(s->nslots);
-  s->d_maps = b.take<BlockRef>(4 * s->cap_map);
-  s->d_ms = s->d_mf = s->d_mr = s->d_slots = s->d_maps;
+  s->d_maps = b.take<BlockRef>(3 * s->cap_map);
+  s->d_ms = s->d_mf = s->d_mr = s->d_maps;
   s->slot_ticket.assign(s->nslots, 0);
   s->slot_rank_cap.assign(s->nslots, 0);
   s->slot_shape.resize(s->nslots);
@@ -600,6 +605,9 @@ int jssp_server_create(jssp_engine* e, const jssp_server_config* cfg, jssp_serve
     s->status_ticket[k].assign(s->nslots, 0);
     if (hipHostMalloc((void**)&s->status[k], (size_t)s->nslots * sizeof(int), hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&s->status_dev[k], s->status[k], 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->fin_map[k], (size_t)s->nslots * sizeof(BlockRef), hipHostMallocCoherent) !=
+            hipSuccess ||
+        hipHostGetDevicePointer((void**)&s->fin_map_dev[k], s->fin_map[k], 0) != hipSuccess ||
         hipEventCreateWithFlags(&s->status_ev[k], hipEventDisableTiming) != hipSuccess) {
       (void)hipGetLastError();
       jssp_server_destroy(s);
@@ -644,6 +652,7 @@ void jssp_server_destroy(jssp_server* s) {
     if (s->stage[k].ev) (void)hipEventDestroy(s->stage[k].ev);
     if (s->stage[k].h) (void)hipHostFree(s->stage[k].h);
     if (s->status[k]) (void)hipHostFree(s->status[k]);
+    if (s->fin_map[k]) (void)hipHostFree(s->fin_map[k]);
   }
   if (s->arena) (void)hipFree(s->arena);
   if (s->meta) (void)hipFree(s->meta);
@@ -732,16 +741,26 @@ int jssp_serve_step(jssp_server* s, int32_t levels, int32_t* active) {
     const int par = s->parity;
     s->parity ^= 1;
     s->status_ticket[par] = s->slot_ticket;
-    // Requests whose last level runs in this step get their jump table, so
-    // the backtrack walks their trail in parallel 32-level groups.
+    // Requests whose last level runs in this step get their jump table (so
+    // the backtrack walks their trail in parallel 32-level groups) and their
+    // backtrack; a step that ends no request launches neither.  The map
+    // buffer of this parity was last read by the step before the previous
+    // one, whose status the previous serve step waited for.
+    std::vector<int>& fin = s->status_fin[par];
+    fin.clear();
     for (int i = 0; i < s->nslots; ++i) {
       if (!s->slot_ticket[i]) continue;
       const Shape& sh = s->slot_shape[i];
-      if (s->slot_level[i] < sh.L && s->slot_level[i] + levels >= sh.L)
+      if (s->slot_level[i] < sh.L && s->slot_level[i] + levels >= sh.L) {
         JSSP_HIP(e, jssp::launch_step_jump(s->d_desc + i, sh.K, sh.L, e->stream));
+        s->fin_map[par][fin.size()] = make_int2(i, 0);
+        fin.push_back(i);
+      }
       s->slot_level[i] = std::min(sh.L, s->slot_level[i] + levels);
     }
-    JSSP_HIP(e, jssp::launch_step_backtrack(s->d_desc, s->d_slots, s->nact, e->stream, s->status_dev[par]));
+    if (!fin.empty())
+      JSSP_HIP(e, jssp::launch_step_backtrack(s->d_desc, s->fin_map_dev[par], (int)fin.size(), e->stream,
+                                              s->status_dev[par]));
     JSSP_HIP(e, hipEventRecord(s->status_ev[par], e->stream));
     s->status_pending = par;
   }
@@ -807,7 +826,7 @@ int jssp_serve_drain(jssp_server* s) {
   std::lock_guard<std::recursive_mutex> lk(s->e->mu);
   for (;;) {
     int active = 0;
-    int rc = jssp_serve_step(s, 16, &active);
+    int rc = jssp_serve_step(s, 32, &active);
     if (rc) return rc;
     bool queued = false;
     for (uint64_t t : s->queue)
