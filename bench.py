@@ -114,10 +114,32 @@ def multiseed(eng, bs, inst, K, seed, beams):
     plain = [eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed + i) for i in range(beams)]
     t_plain = time.perf_counter() - t0
     best = min(p.makespan for p in plain)
-    return {"workload": f"PMS_GPU 50x20 K={K}, {beams} beams (seeds {seed}..{seed + beams - 1}), one device bound",
-            "beams_per_s": round(beams / t_multi, 2), "plain_beams_per_s": round(beams / t_plain, 2),
-            "speedup": round(t_plain / t_multi, 2), "makespan": r.makespan, "plain_best_makespan": best,
-            "equal": r.makespan == best, "expanded_states_per_s": round(r.expanded_states / t_multi, 1)}
+    out = {"workload": f"PMS_GPU 50x20 K={K}, {beams} beams (seeds {seed}..{seed + beams - 1}), one device bound",
+           "beams_per_s": round(beams / t_multi, 2), "plain_beams_per_s": round(beams / t_plain, 2),
+           "speedup": round(t_plain / t_multi, 2), "makespan": r.makespan, "plain_best_makespan": best,
+           "equal": r.makespan == best, "expanded_states_per_s": round(r.expanded_states / t_multi, 1)}
+    # Two passes of beams/2: the second pass prunes with the first pass's
+    # best (the device bound read at init), against the same loop run one
+    # solve at a time with the bound passed as best_known.
+    half = beams // 2
+    eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed, n_seeds=half, n_passes=2)  # warm-up
+    t0 = time.perf_counter()
+    r2 = eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed, n_seeds=half, n_passes=2)
+    t_two = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    p1 = [eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed + i) for i in range(half)]
+    b1 = min(p.makespan for p in p1)
+    p2 = [eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=seed + half + i, best_known=b1) for i in range(half)]
+    t_two_plain = time.perf_counter() - t0
+    b2 = min([b1] + [p.makespan for p in p2 if p.found])
+    unbounded = sum(p.expanded_states for p in plain[half:beams])
+    out["two_passes"] = {
+        "workload": f"{half} beams per pass, 2 passes; pass 2 prunes with pass 1's best ({b1})",
+        "beams_per_s": round(beams / t_two, 2), "plain_beams_per_s": round(beams / t_two_plain, 2),
+        "speedup": round(t_two_plain / t_two, 2), "makespan": r2.makespan, "plain_best_makespan": b2,
+        "equal": r2.makespan == b2,
+        "pass2_expanded_with_bound": sum(p.expanded_states for p in p2), "pass2_expanded_without_bound": unbounded}
+    return out
 
 
 def batch_workload(bs, count=256, gen_seed=2026):
@@ -385,14 +407,28 @@ def cpu_small_config(bs):
 
 
 def _cpu_model():
+    """The host CPU as /proc/cpuinfo names it (the first processor): model
+    name, family / model / stepping, clock, and the logical CPUs this process
+    may run on."""
+    fields = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break
+                k, _, v = line.partition(":")
+                fields[k.strip()] = v.strip()
     except OSError:
+        return "unknown"
+    name = fields.get("model name", "unknown")
+    extra = [f"{k} {fields[k]}" for k in ("cpu family", "model", "stepping") if k in fields]
+    if "cpu MHz" in fields:
+        extra.append(f"{float(fields['cpu MHz']):.0f} MHz")
+    try:
+        extra.append(f"{len(os.sched_getaffinity(0))} logical CPUs available")
+    except (AttributeError, OSError):
         pass
-    return "unknown"
+    return f"{name} ({', '.join(extra)})" if extra else name
 
 
 def main():
