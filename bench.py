@@ -157,6 +157,7 @@ def batched(eng, bs, count, use_graph=True):
     t0 = time.perf_counter()
     rb = eng.solve_batch(insts, params, use_graph=use_graph)
     t_batch = time.perf_counter() - t0
+    log("batched: the same instances one at a time")
     t0 = time.perf_counter()
     rp = [eng.solve(i, p["beam_width"], mode=p["mode"], seed=p["seed"]) for i, p in zip(insts, params)]
     t_plain = time.perf_counter() - t0
@@ -252,11 +253,16 @@ def serving(eng, bs, count, budget_gib, load, sweep=(0.5, 1.0, 4.0)):
         plain.append(eng.solve(inst, K, mode=bs.MODE_PMS_GPU, seed=12345))
         dur.append(time.perf_counter() - t0)
     mean_service = sum(dur) / len(dur)
+    log(f"serving: plain service times measured ({mean_service * 1e3:.1f} ms mean); Poisson run at {load}x")
     main = serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, load)
-    runs = [serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, x) for x in sweep]
+    runs = []
+    for x in sweep:
+        log(f"serving: Poisson run at {x}x")
+        runs.append(serve_poisson(eng, bs, reqs, gaps, plain, dur, budget_gib, x))
     eq = main["identical_results"] and all(r["identical_results"] for r in runs)
     # Saturation: every request queued at t=0, so the server's capacity shows
     # rather than the arrival rate (the plain loop's capacity is 1/mean service).
+    log("serving: every request queued at once")
     srv = bs.Server(eng, int(budget_gib * 2**30), max_slots=64)
     t0 = time.perf_counter()
     tk = [srv.submit(inst, K, mode=bs.MODE_PMS_GPU, seed=12345) for inst, K in reqs]
@@ -452,6 +458,9 @@ def main():
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--serve-load", type=float, default=2.0)
     ap.add_argument("--budget-gib", type=float, default=8.0)
+    ap.add_argument("--tune-table", default=None, metavar="PATH",
+                    help="batch-shape tuning table to load (and extend) instead of tuning in memory; a "
+                         "profile of the batch can then leave the tuning runs out")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (reference point for the "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -475,6 +484,8 @@ def main():
     import beamsearch_jssp_gpu_amd as bs
     if args.batch_only:
         eng = bs.Engine(0)
+        if args.tune_table:
+            eng.set_autotune(True, args.tune_table)
         out = batched(eng, bs, args.batch, use_graph=not args.no_graph)
         eng.close()
         print(json.dumps({"batched": out, "use_graph": not args.no_graph}), flush=True)
