@@ -123,6 +123,8 @@ _SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_double)]),
     "jssp_engine_level_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint32),
                                                ctypes.c_int32]),
+    "jssp_engine_kernel_launches": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_float),
+                                                   ctypes.POINTER(ctypes.c_double), ctypes.c_int32]),
     "jssp_server_create": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_ServerConfig),
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "jssp_server_destroy": (None, [ctypes.c_void_p]),
@@ -395,6 +397,19 @@ class Engine:
         self._check(self.lib.jssp_engine_kernel_time(self.h, kind, ctypes.byref(ms), ctypes.byref(n),
                                                      ctypes.byref(b)))
         return ms.value, n.value, b.value
+
+    def kernel_launches(self, kind: int):
+        """Per launch of `kind` in the last profiled solve, in level order:
+        (durations in ms, algorithmic bytes) as numpy arrays."""
+        n = self.lib.jssp_engine_kernel_launches(self.h, kind, None, None, 0)
+        if n < 0:
+            raise ValueError(f"kernel kind {kind} is not 0, 1 or 2")
+        ms = np.zeros(n, dtype=np.float32)
+        b = np.zeros(n, dtype=np.float64)
+        if n:
+            self.lib.jssp_engine_kernel_launches(self.h, kind, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                                 b.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        return ms, b
 
 
 _default_engine: Optional[Engine] = None

@@ -348,11 +348,15 @@ const EventPair* Prof::pair(int kind, EventPair* ep, bool on) {
   return ep;
 }
 void Prof::collect() {
+  for (auto& v : e->kl_ms) v.clear();
   for (auto& mk : marks) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, mk.second.first, mk.second.second) == hipSuccess) {
       e->kt[mk.first].ms += ms;
       e->kt[mk.first].launches += 1;
+      e->kl_ms[mk.first].push_back(ms);
+    } else {
+      e->kl_ms[mk.first].push_back(-1.f);
     }
   }
 }
@@ -388,16 +392,21 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
   // materialisation reads a parent and the survivor's 8-byte record and
   // writes a child snapshot.
   const double S4 = 4.0 * s.S;
+  for (auto& v : e->kl_bytes) v.clear();
+  auto add = [&](int kind, double b) {
+    e->kt[kind].bytes += b;
+    e->kl_bytes[kind].push_back(b);
+  };
   // Fused: the rank kernel of level l also scores level l + 1 from the
   // survivors' words in registers (8 bytes per child key, no parent read).
   for (int l = 0; l < s.L; ++l) {
     const double P = pc[l], C = lexp[l];
     const double Kp = (double)pc[l + 1];
-    if (s.single) e->kt[kKindScore].bytes += S4 * P + (S4 + 8.0) * C;
-    else if (!s.fused) e->kt[kKindScore].bytes += S4 * P + 8.0 * C;
+    if (s.single) add(kKindScore, S4 * P + (S4 + 8.0) * C);
+    else if (!s.fused) add(kKindScore, S4 * P + 8.0 * C);
     if (l == s.L - 1) break;
-    e->kt[kKindFilter].bytes += 8.0 * C + 8.0 * Kp;
-    e->kt[kKindRank].bytes += 2.0 * S4 * Kp + 8.0 * Kp + (s.fused ? 8.0 * lexp[l + 1] : 0.0);
+    add(kKindFilter, 8.0 * C + 8.0 * Kp);
+    add(kKindRank, 2.0 * S4 * Kp + 8.0 * Kp + (s.fused ? 8.0 * lexp[l + 1] : 0.0));
   }
 }
 
@@ -712,6 +721,18 @@ int jssp_engine_level_stats(const jssp_engine* e, int32_t level, uint32_t* out, 
   const uint32_t v[10] = {st.V, st.Kp, st.lo, st.s1, st.T1, st.cum_lt, st.t1, st.need, st.sure_ctr, st.tie_ctr};
   for (int i = 0; i < count && i < 10; ++i) out[i] = v[i];
   return JSSP_OK;
+}
+
+int jssp_engine_kernel_launches(const jssp_engine* e, int32_t kind, float* ms, double* bytes, int32_t capacity) {
+  if (!e || kind < 0 || kind >= kNumKinds || capacity < 0) return -1;
+  const auto& t = e->kl_ms[kind];
+  const auto& b = e->kl_bytes[kind];
+  const size_t n = std::min(t.size(), b.size());
+  for (size_t i = 0; i < n && i < (size_t)capacity; ++i) {
+    if (ms) ms[i] = t[i];
+    if (bytes) bytes[i] = b[i];
+  }
+  return (int)n;
 }
 
 int jssp_engine_kernel_time(const jssp_engine* e, int32_t kind, double* total_ms, int64_t* launches,

@@ -35,6 +35,10 @@ struct jssp_engine {
   bool profiling = false;
   std::vector<hipEvent_t> pev;  // event pool for per-kernel timing
   KernelTime kt[kNumKinds];
+  // Per launch of each kind in the last profiled single-beam device solve,
+  // in level order: duration (ms) and algorithmic bytes.
+  std::vector<float> kl_ms[kNumKinds];
+  std::vector<double> kl_bytes[kNumKinds];
   std::vector<jssp::SelState> level_log;  // per-level selection state of the last profiled solve
   // Batched solves: their own arena (kept apart from the single-solve one)
   // and the captured step graph of the last batch layout.

@@ -73,6 +73,23 @@ def profile_kinds(eng, inst, K, mode, phases, seed):
     return kinds
 
 
+def full_width(eng, k):
+    """The kernel's rate over the full-width levels of the last profiled
+    solve (algorithmic bytes at least 95% of the widest level's), beside the
+    mean over all its launches: the narrow first and last levels are mostly
+    fixed per-launch cost."""
+    ms, b = eng.kernel_launches(k)
+    ok = ms > 0
+    if not ok.any():
+        return None
+    wide = ok & (b >= 0.95 * b[ok].max())
+    t = float(ms[wide].sum()) / 1e3
+    gbs = float(b[wide].sum()) / t / 1e9
+    return {"levels": int(wide.sum()), "of": int(ok.sum()), "avg_launch_us": round(t / wide.sum() * 1e6, 3),
+            "algorithmic_bytes_per_launch": round(float(b[wide].mean()), 1), "achieved": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def roofline_obj(kind, phases, traffic=None):
     k, ms, n, b = kind
     avg_s = ms / 1e3 / max(1, n)
@@ -545,6 +562,7 @@ def main():
                                        "algorithmic_GBps": round(k[3] / (k[1] / 1e3) / 1e9, 1) if k[1] else 0}
                     for k in kinds if k[2] > 0},
     }
+    line["roofline"]["full_width_levels"] = full_width(eng, dom[0])
     if not args.no_extras:
         log("headline over instance seeds 1..8")
         rs8 = []
@@ -561,6 +579,7 @@ def main():
         if phases == 2:
             log("single-phase expand kernel")
             line["expand_single_phase"] = roofline_obj(profile_kinds(eng, inst, K, mode, 1, seed)[0], 1, traffic)
+            line["expand_single_phase"]["full_width_levels"] = full_width(eng, 0)
         log("multi-seed: 16 PMS beams")
         line["multiseed"] = multiseed(eng, bs, inst, K, 1, 16)
         log(f"batched: {args.batch} uneven instances")

@@ -276,6 +276,12 @@ int jssp_engine_tuned_shape(const jssp_engine* engine, int32_t n_jobs, int32_t n
 int jssp_engine_set_profiling(jssp_engine* engine, int32_t on);
 int jssp_engine_kernel_time(const jssp_engine* engine, int32_t kind, double* total_ms,
                             int64_t* launches, double* algorithmic_bytes);
+/* Per launch of `kind` in the last profiled single-beam device solve, in
+ * level order: duration in ms (-1 if not measured) and algorithmic bytes,
+ * up to `capacity` entries; returns the number of launches (-1 on bad
+ * arguments).  Shows the full-width levels apart from the narrow ones. */
+int jssp_engine_kernel_launches(const jssp_engine* engine, int32_t kind, float* ms, double* algorithmic_bytes,
+                                int32_t capacity);
 /* Selection diagnostics of the last profiled single-beam device solve, per
  * level: out[0..9] = kept children, survivors, score-bit lower bound, digit
  * shift, threshold bin, keys below it, keys in it, survivors taken from it,
