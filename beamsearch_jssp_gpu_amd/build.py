@@ -31,6 +31,11 @@ HIP_FLAGS = [
     "-ffp-contract=off",
     "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # The command processor loads the first kernel arguments (the descriptor
+    # and block map pointers) into SGPRs before a wave starts, so a kernel's
+    # first descriptor load does not wait for a kernarg load: -0.5 ms per
+    # 1000-level solve.
+    "-mllvm", "-amdgpu-kernarg-preload-count=16",
 ]
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", f"-I{ROCM}/include", "-D__HIP_PLATFORM_AMD__"]
 
