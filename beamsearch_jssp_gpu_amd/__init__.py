@@ -557,9 +557,9 @@ class Server:
             pass
 
     def submit(self, inst: JSSPInstance, beam_width: int, mode: int = MODE_PMS_GPU, best_known: int = NO_MAKESPAN,
-               seed: int = 0) -> int:
+               seed: int = 0, flags: int = 0) -> int:
         ci = inst._c()
-        pr = _params(beam_width, mode, 2, best_known, seed)
+        pr = _params(beam_width, mode, 2, best_known, seed, flags=flags)
         t = ctypes.c_uint64()
         self.engine._check(self.lib.jssp_serve_submit(self.h, ctypes.byref(ci), ctypes.byref(pr), ctypes.byref(t)))
         self._keep[t.value] = inst

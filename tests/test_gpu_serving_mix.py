@@ -1,13 +1,18 @@
 """The serving loop at its own workload: the 64-request mix of the serving
 measurement (n in [10,100], m in [5,20], K in {256,1024,4096}, PMS_GPU,
 seeded draws of beamsearch_jssp_gpu_amd/workloads.py), arriving in seeded
-staggered bursts between serve steps, with a burst deep enough that later
-requests are admitted behind more than eight waiting ones (the batched rank
-form) while others run the per-survivor form.
+staggered bursts between serve steps, so the resident load -- and with it
+the rank block shape the server picks at each map rebuild (batched, one
+survivor per wave, single-solve blocks) -- rises and falls while requests
+are in flight.
 
 Every result must equal the same request solved alone, and the requests
 with K >= 1024 whose oracle results are committed
-(tests/golden/oracle_full_size.json) must equal the oracle."""
+(tests/golden/oracle_full_size.json) must equal the oracle.  A second mix
+crosses every shape threshold in both directions with requests in the
+separate-score form and with the generic selection digit (whose digit
+parameters come from the rank blocks' partials, folded when a request's
+block count changes)."""
 import hashlib
 import json
 import os
@@ -65,4 +70,38 @@ def test_serving_mix_equals_alone_and_oracle(engine):
         _same(got, a)
         sub, adm, done = srv.times(t)
         assert 0 <= sub <= adm <= done
+    srv.close()
+
+
+def test_serving_shape_switches_keep_every_form_exact(engine):
+    """Requests of K=4096 (128 rank blocks each in the batched shape) and
+    K=1024 finish at different steps, so the resident load goes from
+    batched (>= 512 blocks) to one survivor per wave (256-512) to
+    single-solve blocks (<= 256), and two late arrivals bring it back up;
+    fused, separate-score (JSSP_FLAG_UNFUSED) and generic-digit requests
+    (JSSP_FLAG_GENERIC_DIGIT, which reads the rank partials) each equal
+    their single solve."""
+    GEN, UNF = 0x4, 0x8
+    jobs = [  # (n, m, K, flags, gen seed)
+        (30, 10, 4096, 0, 101), (20, 10, 4096, GEN, 102), (15, 5, 4096, UNF, 103),
+        (12, 5, 4096, 0, 104), (10, 5, 4096, GEN, 105), (40, 10, 1024, UNF, 106),
+    ]
+    late = [(20, 10, 4096, GEN, 107), (16, 8, 4096, 0, 108)]
+    insts = [bs.generate_instance(g, n, m, 1, 99) for n, m, K, f, g in jobs + late]
+    params = [(K, f) for n, m, K, f, g in jobs + late]
+    print("shape switches: solving", len(insts), "requests alone", flush=True)
+    alone = [engine.solve(i, K, mode=bs.MODE_PMS_GPU, seed=31 + k, flags=f)
+             for k, (i, (K, f)) in enumerate(zip(insts, params))]
+    srv = bs.Server(engine, hbm_budget_bytes=8 << 30, max_slots=16)
+    tickets = [srv.submit(insts[k], params[k][0], bs.MODE_PMS_GPU, seed=31 + k, flags=params[k][1])
+               for k in range(len(jobs))]
+    for _ in range(15):  # 240 levels: the four shortest requests finish
+        srv.step(16)
+    tickets += [srv.submit(insts[k], params[k][0], bs.MODE_PMS_GPU, seed=31 + k, flags=params[k][1])
+                for k in range(len(jobs), len(insts))]
+    srv.drain()
+    for k, t in enumerate(tickets):
+        r = srv.poll(t)
+        assert r is not None, k
+        _same(r, alone[k])
     srv.close()
