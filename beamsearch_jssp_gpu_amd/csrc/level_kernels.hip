@@ -1144,17 +1144,31 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   const bool subs = sure_subs(single, a.K, a.B);
   const uint32_t sub = (uint32_t)local % kSureSubs;
   uint32_t* sctr = a.sctr + sub * kCtrStride;
+  // A block with several tiles (throughput shapes) loads each next tile's
+  // keys, digits and parent counts while it compacts the current one, so a
+  // tile costs one round trip (its reservation atomic) instead of three; the
+  // loads are speculative within the buffers' K*B capacity and masked when
+  // used, as the first tile's (issued in the prologue).
+  uint32_t c_key = key_first, c_dg = dg_first, c_pv = pv_first;
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
-    const bool first = base == (uint32_t)local * blockDim.x;
-    // The slot's parent count, then its key and digit (the first tile's were
-    // issued in the prologue): invalid slots read neither.
+    const uint32_t xn = x + (uint32_t)nblk * blockDim.x;
+    uint32_t n_key = kNoScore, n_dg = (uint32_t)kNB, n_pv = 0u;
+    if (base + (uint32_t)nblk * blockDim.x < N && xn < cap) {
+      n_key = keys[xn];
+      n_dg = (uint32_t)kdig[xn];
+      n_pv = (uint32_t)pvalid[xn / B];
+    }
+    // A slot holds a key only below its parent's valid count.
     const uint32_t px = x / B;
-    const bool has = inb && (x - px * B) < (first ? pv_first : (uint32_t)pvalid[px]);
-    const uint32_t sb = has ? (first ? key_first : keys[x]) : kNoScore;
+    const bool has = inb && (x - px * B) < c_pv;
+    const uint32_t sb = has ? c_key : kNoScore;
     const u64 key = sb != kNoScore ? (((u64)sb << 32) | x) : ~0ull;
-    const uint32_t dg = has ? (first ? dg_first : (uint32_t)kdig[x]) : (uint32_t)kNB;
+    const uint32_t dg = has ? c_dg : (uint32_t)kNB;
+    c_key = n_key;
+    c_dg = n_dg;
+    c_pv = n_pv;
     const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
     const bool sure = d < T1;
     const bool tie = d == T1;
