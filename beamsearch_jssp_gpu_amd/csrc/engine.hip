@@ -134,6 +134,7 @@ static size_t rank_lds_for(int K, int n, int m, bool fused, int waves) {
 bool fused_eligible(const jssp_instance* in, const jssp_params* p) {
   const bool exact = (p->mode == JSSP_MODE_ICH_EXACT || p->mode == JSSP_MODE_PMS_EXACT);
   return !exact && p->phases == 2 && !(p->flags & (JSSP_FLAG_GENERIC_DIGIT | JSSP_FLAG_UNFUSED)) &&
+         p->beam_width <= kRankLdsMax &&
          rank_lds_for(p->beam_width, in->n_jobs, in->n_machines, true, kRankWaves) <= kRankFusedLdsMax &&
          exact_digit_applies(in);
 }
@@ -313,7 +314,7 @@ size_t filter_lds_bytes(const Shape& s, const jssp_params* p) {
 size_t rank_lds_bytes(const Shape& s) { return rank_lds_for(s.K, s.n, s.m, s.fused, s.rank_waves); }
 
 int rank_variant(const Shape& s, const jssp_params* p) {
-  if (!s.fused) return kRankUnfused;
+  if (!s.fused) return s.K > kRankLdsMax ? kRankUnfusedWide : kRankUnfused;
   const bool small = s.S <= 128 && s.n <= 64;
   if (p->mode == JSSP_MODE_PMS_GPU) return small ? kRankFusedPmsSmall : kRankFusedPms;
   return small ? kRankFusedIchSmall : kRankFusedIch;

@@ -58,6 +58,28 @@ __device__ __forceinline__ GL T* gp(T* p) {
   return (GL T*)p;
 }
 
+// Whole-struct copies to / from global memory through global pointers (a
+// struct's implicit assignment does not take an address-space-qualified
+// object), word by word; the compiler merges the words into wide accesses.
+template <class T>
+__device__ __forceinline__ void store_global(T* dst, const T& v) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  GL uint32_t* d = reinterpret_cast<GL uint32_t*>(gp(dst));
+  const uint32_t* w = reinterpret_cast<const uint32_t*>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) d[i] = w[i];
+}
+template <class T>
+__device__ __forceinline__ T load_global(const T* src) {
+  static_assert(sizeof(T) % 4 == 0, "whole words");
+  GL const uint32_t* s = reinterpret_cast<GL const uint32_t*>(gp(src));
+  T v;
+  uint32_t* w = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) w[i] = s[i];
+  return v;
+}
+
 // Optional phase stamps, compiled in only with -DJSSP_PHASE_STAMPS (a
 // separate build, see beamsearch_jssp_gpu_amd/phase_stamps.py): wall-clock
 // (100 MHz) stamps at phase boundaries of the filter (thread 0 of each block)
@@ -139,6 +161,13 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Device-scope relaxed add on global memory, returning the old value (the
+// same as atomicAdd, through a global pointer so no flat instruction holds
+// up the wave's later LDS waits).
+__device__ __forceinline__ uint32_t dev_add(GL uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Set bits of `mask` below this lane.
@@ -560,7 +589,7 @@ __device__ __forceinline__ void finish_score_block(ScoreAcc acc, const uint32_t*
       b.min_cmax = min(b.min_cmax, s_kmin[w]);
       b.best = s_best[w] < b.best ? s_best[w] : b.best;
     }
-    bstat[slot] = b;
+    store_global(bstat + slot, b);
   }
   if (!last) {
     GL uint32_t* hc = hist + (size_t)(slot % kHistCopies) * kNB;
@@ -587,15 +616,17 @@ __device__ __forceinline__ ScoreLevel score_level(const LevelArgs& a, int local)
   sl.last = (a.level == a.L - 1);
   sl.lo = 0;
   sl.s1 = kLevel0Shift;
+  GL SelState* sel = gp(a.sel);
   if (a.exact_digit) {
-    sl.lo = a.sel->lo;
+    sl.lo = sel->lo;
   } else if (a.level > 0 && !sl.last) {
     int cmin = INT_MAX;
     uint32_t mxh = 0, mnh = 0xFFFFFFFFu;
+    GL const RankStat* rstat = gp(a.rstat);
     for (int i = threadIdx.x; i < a.nblk_rank; i += blockDim.x) {
-      cmin = min(cmin, a.rstat[i].min_cmax);
-      mxh = max(mxh, a.rstat[i].max_hi);
-      mnh = min(mnh, a.rstat[i].min_hi);
+      cmin = min(cmin, rstat[i].min_cmax);
+      mxh = max(mxh, rstat[i].max_hi);
+      mnh = min(mnh, rstat[i].min_hi);
     }
     cmin = wave_min_i(cmin);
     mxh = wave_max(mxh);
@@ -616,9 +647,9 @@ __device__ __forceinline__ ScoreLevel score_level(const LevelArgs& a, int local)
     sl.s1 = (uint32_t)max(0, bl - kSpreadBins);
   }
   if (local == 0 && threadIdx.x == 0) {
-    if (!a.exact_digit) a.sel->lo = sl.lo;
-    a.sel->s1 = sl.s1;
-    a.sel->exact = a.exact_digit ? 1u : 0u;
+    if (!a.exact_digit) sel->lo = sl.lo;
+    sel->s1 = sl.s1;
+    sel->exact = a.exact_digit ? 1u : 0u;
   }
   return sl;
 }
@@ -715,6 +746,12 @@ __device__ __forceinline__ void write_children(const LevelArgs& a, const Layout&
       }
     }
   }
+  // The parent words are used only inside the loop: a wave with no kept
+  // child waits for them here, so no load is left in flight for the block's
+  // tail to wait on (one vector-memory counter also covers the stores, so
+  // such a wait there would hold the histogram flush behind every store).
+#pragma unroll
+  for (int i = 0; i < NWM; ++i) asm volatile("" ::"v"(pw[i]));
 }
 
 __device__ __forceinline__ void write_children_any(const LevelArgs& a, const Layout& Ly, int p, bool pact,
@@ -776,7 +813,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   // arrives (every row below K is allocated; the words are used only when
   // the parent exists).
   ParentWords first = load_parent_words(a, Ly, pb0 + sub, pb0 + sub < a.K, gl, NH);
-  const int P = a.pcount[a.level];
+  const int P = gp(a.pcount)[a.level];
   const ScoreLevel sl = score_level(a, local);
   const ScoreDst dst = score_dst(a);
   GL const int2* ops = gp(a.ops);
@@ -821,6 +858,11 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
     emit_children(dst, sl, p, pact, G, gl, gmask, lane, v, s_h, acc, slot, keep, nvalid);
     if (SINGLE && !sl.last) write_children_any(a, Ly, p, pact, par, v, slot, keep, lane, s_rows + wave * 128);
   }
+  // Loads whose values only some paths use are waited for here (see
+  // write_children): the first parent row of a wave without a parent, the
+  // digit base of the last level.
+  if (pb0 >= P) asm volatile("" ::"v"(first.pc), "v"(first.w[0]), "v"(first.w[1]), "v"(first.pred[0]), "v"(first.pred[1]));
+  asm volatile("" ::"v"(sl.lo));
   finish_score_block(acc, s_h, a.bstat, gp(a.hist), local, sl.last);
 }
 
@@ -1052,11 +1094,12 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_base[2];
   __shared__ int s_last;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  SelState* sel = a.sel;
+  GL SelState* sel = gp(a.sel);
   FILTER_STAMP(0);
   // The next level's histogram (read by the filter one level back) starts at
   // zero before its score pass, which follows this kernel.
-  for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) a.hist_next[i] = 0;
+  GL uint32_t* hist_next = gp(a.hist_next);
+  for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) hist_next[i] = 0;
 
   // Every load of the prologue goes out at once: the histogram copies, one
   // score-block partial per thread (nblk_score <= 512), the level's parents,
@@ -1110,12 +1153,12 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     km = INT_MAX;
     for (int w = 0; w < 16; ++w) { ex += s_e[w]; km = min(km, s_km[w]); }
     // Lower bound of the next level's parents' cmax (they are kept children).
-    if (a.exact_digit && km != INT_MAX) a.sel_next->lo = (uint32_t)km;
+    if (a.exact_digit && km != INT_MAX) gp(a.sel_next)->lo = (uint32_t)km;
     sel->V = v;
     sel->Kp = Kp;
-    a.pcount[a.level + 1] = (int)Kp;
-    *a.expanded += (u64)ex;
-    a.lvl_exp[a.level] = ex;
+    gp(a.pcount)[a.level + 1] = (int)Kp;
+    *gp(a.expanded) += (u64)ex;
+    gp(a.lvl_exp)[a.level] = ex;
   }
   if (Kp == 0) return;
   const bool exact = a.exact_digit != 0;
@@ -1143,7 +1186,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   GL u64* tiles_keys = gp(a.ref_buf);
   const bool subs = sure_subs(single, a.K, a.B);
   const uint32_t sub = (uint32_t)local % kSureSubs;
-  uint32_t* sctr = a.sctr + sub * kCtrStride;
+  GL uint32_t* sctr = gp(a.sctr) + sub * kCtrStride;
   // A block with several tiles (throughput shapes) loads each next tile's
   // keys, digits and parent counts while it compacts the current one, so a
   // tile costs one round trip (its reservation atomic) instead of three; the
@@ -1185,10 +1228,10 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
         s_off[0][w] = ts; s_off[1][w] = tt;
         ts += cs; tt += ct;
       }
-      if (subs) s_base[0] = ts ? sub * (uint32_t)a.K + atomicAdd(sctr, ts) : 0u;
-      else s_base[0] = ts ? atomicAdd(&sel->sure_ctr, ts) : 0u;
+      if (subs) s_base[0] = ts ? sub * (uint32_t)a.K + dev_add(sctr, ts) : 0u;
+      else s_base[0] = ts ? dev_add(&sel->sure_ctr, ts) : 0u;
       if (single) gp(a.tcnt)[base / blockDim.x] = tt;
-      else s_base[1] = tt ? cum_lt + atomicAdd(&sel->tie_ctr, tt) : 0u;
+      else s_base[1] = tt ? cum_lt + dev_add(&sel->tie_ctr, tt) : 0u;
     }
     lds_barrier();
     if (sure) cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
@@ -1211,7 +1254,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __syncthreads();
   if (tid == 0) {
     __threadfence();
-    const uint32_t ticket = atomicAdd(&sel->done_ctr, 1u);
+    const uint32_t ticket = dev_add(&sel->done_ctr, 1u);
     s_last = (ticket == (uint32_t)nblk - 1) ? 1 : 0;
   }
   __syncthreads();
@@ -1394,7 +1437,7 @@ __device__ __forceinline__ void count_below_n(const u64* C, int n, const u64 (&k
 // SMALL (fused only): S <= 128 and n <= 64, so a snapshot is two words per
 // lane and a job one slot per lane.  W: waves per block (kRankWaves for
 // single solves, kRankWavesThroughput for batches and the server).
-template <bool FUSED, int MODE, bool SMALL, int W, int NB>
+template <bool FUSED, int MODE, bool SMALL, int W, int NB, bool WIDE = false>
 __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nblk) {
   constexpr int NWM = SMALL ? 2 : kMaxWPL;
   constexpr int NHM = SMALL ? 1 : 2;
@@ -1413,12 +1456,12 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     sn->tie_ctr = 0;
     sn->done_ctr = 0;
     for (int r = 0; r < kSureSubs; ++r) gp(a.sctr_next)[r * kCtrStride] = 0u;
-    if (a.sel_log) a.sel_log[a.level] = *a.sel;
+    if (a.sel_log) store_global(a.sel_log + a.level, load_global(a.sel));
   }
   // The selection state, this thread's first tie count and the survivor keys
   // are loaded together; the keys go to LDS whatever the survivor count
   // (cand holds K*B entries, and entries past the survivors are never read).
-  const bool in_lds = a.K <= kRankLdsMax;
+  const bool in_lds = !WIDE;
   const int tiles_max = (int)(((uint32_t)a.K * (uint32_t)a.B + kFilterThreads - 1) / kFilterThreads);
   const int per = (tiles_max + (int)blockDim.x - 1) / (int)blockDim.x;  // tiles per thread (<= 8)
   const int t0 = min(tiles_max, tid * per);
@@ -1619,7 +1662,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
       rs.max_hi = max(rs.max_hi, s_mh[w]);
       rs.min_hi = min(rs.min_hi, s_ml[w]);
     }
-    a.rstat[local] = rs;
+    store_global(a.rstat + local, rs);
   }
   RANK_STAMP(5);
   if (FUSED) finish_score_block(acc, s_h, a.bstat, gp(a.hist_next), local, sln.last);
@@ -1889,9 +1932,10 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
 }
 
 // Advances a level counter (block 0 of an instance, after its reader kernel
-// has completed).
+// has completed).  A global store: a flat one would also count against the
+// wave's LDS waits until it completes.
 __device__ __forceinline__ void advance(int& counter, int cur, int local, int L) {
-  if (local == 0 && threadIdx.x == 0 && cur < L) counter = cur + 1;
+  if (local == 0 && threadIdx.x == 0 && cur < L) *gp(&counter) = cur + 1;
 }
 
 // Two-phase score: keys and selection digits only.
@@ -1943,18 +1987,18 @@ __global__ __launch_bounds__(kFilterThreads, 8) void k_filter_tp_b(InstDesc* des
 
 // Rank + materialise; instances of the other form return at once (a batch
 // mixing both forms launches both kernels over the same map).
-template <int W, int NB>
+template <int W, int NB, bool WIDE>
 __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_rank_b(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
-  if (d.fused) return;
+  if ((d.fused != 0) | ((d.base.K > kRankLdsMax) != WIDE)) return;
   const int level = d.lvl_rank;
   // descriptor fields are read before the counter store (scalar loads)
   const LevelArgs a = resolve(d, level);
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<false, 0, false, W, NB>(a, nullptr, r.y, nblk);
+  rank_body<false, 0, false, W, NB, WIDE>(a, nullptr, r.y, nblk);
 }
 
 // Rank + materialise + score of the next level (fused form), per mode.
@@ -2130,7 +2174,8 @@ static void (*rank_kernel(int variant))(InstDesc*, const BlockRef*) {
     case kRankFusedPms: return k_rank_fused_b<2, false, W, NB>;
     case kRankFusedIchSmall: return k_rank_fused_b<1, true, W, NB>;
     case kRankFusedPmsSmall: return k_rank_fused_b<2, true, W, NB>;
-    default: return k_rank_b<W, NB>;
+    case kRankUnfusedWide: return k_rank_b<W, NB, true>;
+    default: return k_rank_b<W, NB, false>;
   }
 }
 

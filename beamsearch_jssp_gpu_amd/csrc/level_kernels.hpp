@@ -178,7 +178,8 @@ enum RankVariant : int {
   kRankFusedPms = 2,
   kRankFusedIchSmall = 3,
   kRankFusedPmsSmall = 4,
-  kRankVariants = 5
+  kRankUnfusedWide = 5,  // separate-score form with K > kRankLdsMax (survivors ranked from global memory)
+  kRankVariants = 6
 };
 // waves: kRankWaves or kRankWavesThroughput (the Shape's rank_waves);
 // rank_batch > 1 (throughput waves only): the kernels that rank kRankBatch
