@@ -665,7 +665,9 @@ __device__ __forceinline__ ScoreLevel score_level(const LevelArgs& a, int local)
 // kept children's fields are compacted into the wave's LDS rows (finish,
 // cmax, new job word, slot | machine << 8 | position << 16 | job << 24) and
 // read back kWriteUnroll at a time by every lane (broadcast reads), so
-// several children's stores are in flight per wave.
+// several children's stores are in flight per wave.  (Loading the parent's
+// words with its first loads instead, so that the first child store does
+// not wait for them, measured no faster.)
 // NWM: words per lane the caller supports (>= child_stride(S) / 64).
 constexpr int kWriteUnroll = 4;
 
@@ -748,8 +750,9 @@ __device__ __forceinline__ void write_children(const LevelArgs& a, const Layout&
   }
   // The parent words are used only inside the loop: a wave with no kept
   // child waits for them here, so no load is left in flight for the block's
-  // tail to wait on (one vector-memory counter also covers the stores, so
-  // such a wait there would hold the histogram flush behind every store).
+  // tail to wait on (gfx9's one vector-memory counter also counts the
+  // stores, so a wait there would hold the histogram flush behind every
+  // child store).
 #pragma unroll
   for (int i = 0; i < NWM; ++i) asm volatile("" ::"v"(pw[i]));
 }
@@ -818,7 +821,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   const ScoreDst dst = score_dst(a);
   GL const int2* ops = gp(a.ops);
   for (int i = threadIdx.x; i < kNB; i += blockDim.x) s_h[i] = 0;
-  __syncthreads();
+  lds_barrier();  // the histogram (LDS); the parent loads stay in flight
   ScoreAcc acc;
 
   for (int pb = pb0; pb < P; pb += nblk * nwaves * PPW) {
@@ -858,9 +861,9 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
     emit_children(dst, sl, p, pact, G, gl, gmask, lane, v, s_h, acc, slot, keep, nvalid);
     if (SINGLE && !sl.last) write_children_any(a, Ly, p, pact, par, v, slot, keep, lane, s_rows + wave * 128);
   }
-  // Loads whose values only some paths use are waited for here (see
-  // write_children): the first parent row of a wave without a parent, the
-  // digit base of the last level.
+  // Loads whose values only some paths use are waited for here, so none is
+  // left in flight for the block's tail (see write_children): the first
+  // parent of a wave without a parent, the digit base of the last level.
   if (pb0 >= P) asm volatile("" ::"v"(first.pc), "v"(first.w[0]), "v"(first.w[1]), "v"(first.pred[0]), "v"(first.pred[1]));
   asm volatile("" ::"v"(sl.lo));
   finish_score_block(acc, s_h, a.bstat, gp(a.hist), local, sl.last);
@@ -1293,9 +1296,28 @@ __device__ __forceinline__ void load_parent(const LevelArgs& a, const Layout& Ly
   }
 }
 
+// Writes survivor q's words nw (nw[i] = word lane + 64 i) to the next beam
+// and its trail entry (parent p, job j).  One wave.
+template <int NWM = kMaxWPL>
+__device__ __forceinline__ void store_child(const LevelArgs& a, const Layout& Ly, int lane, int p, int j, int q,
+                                            const int (&nw)[kMaxWPL]) {
+  const int S = Ly.S;
+  const int NW = (S + 63) >> 6;
+  GL int* out = gp(a.beam_next) + (size_t)q * S;
+#pragma unroll
+  for (int i = 0; i < kMaxWPL; ++i) {
+    const int o = lane + 64 * i;
+    if (i < NWM && i < NW && o < S) out[o] = nw[i];
+  }
+  if (lane == 0) gp(a.trail)[(size_t)a.level * a.K + q] = (p << 8) | j;
+}
+
 // Builds survivor q of the next beam from its parent p's words pw and the
 // child's record (two-phase), writes it and the trail entry.  One wave.
-// Returns the child's cmax; nw[i] = the child's word lane + 64 i.
+// Returns the child's cmax; nw[i] = the child's word lane + 64 i.  (Storing
+// the words after the next level's op-table loads are issued, so that the
+// wait for those loads does not also wait for the stores, measured no
+// faster for single solves and slower for the server.)
 template <int ORDER, int NWM = kMaxWPL>
 __device__ __forceinline__ int build_child(const LevelArgs& a, const Layout& Ly, int lane, int p, u64 rec, int q,
                                            const int (&pw)[kMaxWPL], int (&nw)[kMaxWPL]) {
@@ -1329,18 +1351,16 @@ __device__ __forceinline__ int build_child(const LevelArgs& a, const Layout& Ly,
   c.spos = jw_pos(jwj);
   c.jlast = jlast;
   c.newjw = nx + 1 < m ? jw_make(nx + 1, r - 1, rec_next_machine(rec)) : jw_make(nx + 1, kFinishedPos, 0);
-  GL int* out = gp(a.beam_next) + (size_t)q * S;
 #pragma unroll
   for (int i = 0; i < kMaxWPL; ++i) {
     if (i < NWM && i < NW) {
       const int o = lane + 64 * i;
       nw[i] = child_word<ORDER>(o, pw[i], c, Ly);
-      if (o < S) out[o] = nw[i];
     } else {
       nw[i] = 0;
     }
   }
-  if (lane == 0) gp(a.trail)[(size_t)a.level * a.K + q] = (p << 8) | j;
+  store_child<NWM>(a, Ly, lane, p, j, q, nw);
   return c.cm;
 }
 
