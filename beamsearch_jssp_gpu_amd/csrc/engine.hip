@@ -181,7 +181,6 @@ size_t carve(const Shape& s, char* base, Work& w) {
   w.crec[1] = b.take<unsigned long long>(N);
   w.cand = b.take<unsigned long long>(N);
   w.ref_buf = b.take<unsigned long long>(N + s.K);
-  w.kdig = b.take<uint16_t>(N);
   w.pvalid = b.take<uint8_t>(s.K);
   w.tcnt = b.take<uint32_t>((N + kFilterThreads - 1) / kFilterThreads + 2);
   w.hist[0] = b.take<uint32_t>((size_t)kNB * kHistCopies);
@@ -212,16 +211,31 @@ size_t carve(const Shape& s, char* base, Work& w) {
   return b.off + 256;
 }
 
-// The exact selection digit is valid when, at every level, a lower makespan
-// always gives a lower FP32 score: for cmax c1 < c2 at ns scheduled ops,
-// score(c1) <= c1/ns + 1e-6 + ulp and score(c2) >= c2/ns - ulp/2, so it
-// suffices that 1/L > 1e-6 + 2 ulp(2 dmax + 1), since every score is below
-// dmax + 1e-6 (a semi-active schedule grows cmax by at most one duration per
-// op, so cmax/ns <= dmax).
+// The exact selection digit and its key word (cmax << 11 | the score's ulps
+// above fl(cmax / ns)) are valid when
+// (1) at every level a lower makespan always gives a lower FP32 score: for
+//     cmax c1 < c2 at ns scheduled ops, score(c1) <= c1/ns + 1e-6 + ulp and
+//     score(c2) >= c2/ns - ulp/2, so it suffices that 1/L > 1e-6 +
+//     2 ulp(2 dmax + 1), since every score is below dmax + 1e-6 (a
+//     semi-active schedule grows cmax by at most one duration per op, so
+//     cmax/ns <= dmax); the key word then orders children as the score does;
+// (2) cmax fits 21 bits below the all-ones word: cmax <= the sum of all
+//     durations (every start is a finish time of an earlier op);
+// (3) the noise offset fits 11 bits: the busiest machine has at least the
+//     average load, so cmax >= ns dmin / m and fl(cmax / ns) >= 2^-7 when
+//     128 dmin >= m; an ulp there is at least 2^-30 and the noise (below
+//     1e-6) adds at most 1075 ulps.
 bool exact_digit_applies(const jssp_instance* in) {
-  int dmax = 0;
-  for (int i = 0; i < in->n_jobs * in->n_machines; ++i) dmax = std::max(dmax, (int)in->duration[i]);
-  if (dmax <= 0) return false;
+  int dmax = 0, dmin = INT32_MAX;
+  long long dsum = 0;
+  for (int i = 0; i < in->n_jobs * in->n_machines; ++i) {
+    dmax = std::max(dmax, (int)in->duration[i]);
+    dmin = std::min(dmin, (int)in->duration[i]);
+    dsum += in->duration[i];
+  }
+  if (dmax <= 0 || dmin <= 0) return false;
+  if (dsum >= (1ll << (32 - 11)) - 1) return false;
+  if (128ll * dmin < in->n_machines) return false;
   int e = 0;
   std::frexp(2.0 * dmax + 1.0, &e);  // 2 dmax + 1 in [2^(e-1), 2^e)
   const double ulp = std::ldexp(1.0, (e - 1) - 23);
@@ -240,7 +254,6 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.crec = w.crec[0];
   a.cand = w.cand;
   a.ref_buf = w.ref_buf;
-  a.kdig = w.kdig;
   a.pvalid = w.pvalid;
   a.tcnt = w.tcnt;
   a.hist = w.hist[0];
@@ -387,11 +400,13 @@ namespace {
 
 void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int>& pc,
                            const std::vector<uint32_t>& lexp) {
-  // Algorithmic bytes per launch: the parent snapshots read (4S each), the
-  // children written (4S each, single-phase) and the 8-byte (score, child)
-  // keys; the filter reads every key once and writes the survivors;
-  // materialisation reads a parent and the survivor's 8-byte record and
-  // writes a child snapshot.
+  // Algorithmic bytes per launch: a score pass reads the parent snapshots
+  // (4S each) and writes, per child, its 4-byte key word and its 8-byte
+  // record (and, single-phase, its 4S-byte snapshot); the filter reads every
+  // key word once and writes the survivors' 8-byte (key word, child) keys;
+  // materialisation reads each survivor's key, its parent and its record and
+  // writes its snapshot.  C counts the children evaluated (every valid slot;
+  // the pruned ones get a key word and no record, which the count includes).
   const double S4 = 4.0 * s.S;
   for (auto& v : e->kl_bytes) v.clear();
   auto add = [&](int kind, double b) {
@@ -399,15 +414,15 @@ void add_algorithmic_bytes(jssp_engine* e, const Shape& s, const std::vector<int
     e->kl_bytes[kind].push_back(b);
   };
   // Fused: the rank kernel of level l also scores level l + 1 from the
-  // survivors' words in registers (8 bytes per child key, no parent read).
+  // survivors' words in registers (12 bytes per child, no parent read).
   for (int l = 0; l < s.L; ++l) {
     const double P = pc[l], C = lexp[l];
     const double Kp = (double)pc[l + 1];
-    if (s.single) add(kKindScore, S4 * P + (S4 + 8.0) * C);
-    else if (!s.fused) add(kKindScore, S4 * P + 8.0 * C);
+    if (s.single) add(kKindScore, S4 * P + (S4 + 12.0) * C);
+    else if (!s.fused) add(kKindScore, S4 * P + 12.0 * C);
     if (l == s.L - 1) break;
-    add(kKindFilter, 8.0 * C + 8.0 * Kp);
-    add(kKindRank, 2.0 * S4 * Kp + 8.0 * Kp + (s.fused ? 8.0 * lexp[l + 1] : 0.0));
+    add(kKindFilter, 4.0 * C + 8.0 * Kp);
+    add(kKindRank, 2.0 * S4 * Kp + 8.0 * Kp + (s.fused ? 12.0 * lexp[l + 1] : 0.0));
   }
 }
 

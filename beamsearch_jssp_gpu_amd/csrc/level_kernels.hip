@@ -228,6 +228,21 @@ __device__ __forceinline__ uint32_t digit1(uint32_t hi, uint32_t lo, uint32_t s1
   return d < (uint32_t)kNB ? d : (uint32_t)(kNB - 1);
 }
 
+// Exact-digit key word of a child: its cmax above 11 bits of noise offset
+// (the score's ulps above fl(cmax / ns)); and its first selection digit,
+// 8 * min(cmax - lo, 256) + min(offset, 7), lo a lower bound of the level's
+// cmax.  Monotone non-decreasing in the key word, and every bin other than
+// the clamp bins (offset 7, or cmax - lo >= 256) holds one key word.
+constexpr int kOffBits = 11;
+__device__ __forceinline__ uint32_t exact_key(int cmax, uint32_t off) {
+  return ((uint32_t)cmax << kOffBits) | off;
+}
+__device__ __forceinline__ uint32_t exact_digit(uint32_t kw, uint32_t lo) {
+  const uint32_t dc = min((kw >> kOffBits) - lo, 256u);
+  const uint32_t off = min(kw & ((1u << kOffBits) - 1u), 7u);
+  return min(8u * dc + off, (uint32_t)(kNB - 1));
+}
+
 // What changes from a parent snapshot to one child.
 struct ChildMod {
   int mc, f, j, cm, spos, jlast, newjw;
@@ -339,7 +354,6 @@ constexpr uint32_t kNoScore = 0xFFFFFFFFu;  // no finite positive float has thes
 
 struct ScoreDst {
   GL uint32_t* keys;
-  GL uint16_t* kdig;
   GL uint8_t* pvalid;
   GL u64* crec;
   uint32_t lseed;
@@ -348,7 +362,6 @@ struct ScoreDst {
 __device__ __forceinline__ ScoreDst score_dst(const LevelArgs& a) {
   ScoreDst d;
   d.keys = reinterpret_cast<GL uint32_t*>(gp(a.keys));
-  d.kdig = gp(a.kdig);
   d.pvalid = gp(a.pvalid);
   d.crec = gp(a.crec);
   d.lseed = a.lseed;
@@ -538,18 +551,19 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
           const float base = gpu_score_base(v.cm[h], sl.ns);
           const float sc = gpu_score_add_noise(base, noise_hash(a.lseed, (uint32_t)p, (uint32_t)slot[h]));
           const uint32_t bits = __float_as_uint(sc);
-          a.keys[x] = bits;
           if (a.exact_digit) {
-            // Monotone in (score, child) when a lower cmax always scores
-            // lower (checked per instance on the host): cmax >= lo, and for
-            // one cmax the score grows with its ulp offset.
-            const uint32_t dc = min((uint32_t)(v.cm[h] - (int)sl.lo), 256u);
-            const uint32_t off = min(bits - __float_as_uint(base), 7u);
-            d = min(8u * dc + off, (uint32_t)(kNB - 1));
+            // Key word (cmax << 11 | noise offset in ulps): order-isomorphic
+            // to the score bits when a lower cmax always scores lower and the
+            // offset stays below 2047 (both checked per instance on the host),
+            // and it carries what the exact digit needs, so the filter reads
+            // no digit array.
+            const uint32_t kw = exact_key(v.cm[h], bits - __float_as_uint(base));
+            a.keys[x] = kw;
+            d = exact_digit(kw, sl.lo);
           } else {
+            a.keys[x] = bits;
             d = digit1(bits, sl.lo, sl.s1);
           }
-          a.kdig[x] = (uint16_t)d;
           counted = true;
           acc.cnt++;
           acc.kmin = min(acc.kmin, v.cm[h]);
@@ -1104,20 +1118,21 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   GL uint32_t* hist_next = gp(a.hist_next);
   for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) hist_next[i] = 0;
 
-  // Every load of the prologue goes out at once: the histogram copies, one
-  // score-block partial per thread (nblk_score <= 512), the level's parents,
-  // and this block's first tile of keys and digits (read within the buffers'
-  // K*B capacity; slots past this level's count are ignored below).
-  // A slot x = p*B + s holds a key only below its parent's valid count
-  // pvalid[p]; the first tile's keys are read speculatively and masked.
+  // Every load of the prologue goes out at once: the digit parameters, the
+  // histogram copies, one score-block partial per thread (nblk_score <=
+  // 512), the level's parents, and this block's first tile of key words
+  // (read within the buffers' K*B capacity; slots past this level's count
+  // are ignored below).  A slot x = p*B + s holds a key only below its
+  // parent's valid count pvalid[p]; the first tile's keys are read
+  // speculatively and masked.  A key's digit is computed from its key word.
   GL const uint32_t* keys = reinterpret_cast<GL const uint32_t*>(gp(a.keys));
-  GL const uint16_t* kdig = gp(a.kdig);
   GL const uint8_t* pvalid = gp(a.pvalid);
+  // (uniform: held in scalar registers, the throughput form is short of vector ones)
+  const uint32_t dlo = __builtin_amdgcn_readfirstlane(sel->lo), ds1 = __builtin_amdgcn_readfirstlane(sel->s1);
   const uint32_t B = (uint32_t)a.B;
   const uint32_t cap = (uint32_t)a.K * B;
   const uint32_t xf = (uint32_t)local * blockDim.x + tid;
   const uint32_t key_first = xf < cap ? keys[xf] : kNoScore;
-  const uint32_t dg_first = xf < cap ? (uint32_t)kdig[xf] : (uint32_t)kNB;
   const uint32_t pf = xf / B;
   const uint32_t pv_first = xf < cap ? (uint32_t)pvalid[pf] : 0u;
   uint32_t h0 = 0, h1 = 0;
@@ -1165,7 +1180,7 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   }
   if (Kp == 0) return;
   const bool exact = a.exact_digit != 0;
-  const uint32_t s1 = exact ? 0u : sel->s1;
+  const uint32_t s1 = exact ? 0u : ds1;
   const uint32_t ex0 = wofs + inc - sum;
   s_ex[2 * tid] = ex0;
   s_ex[2 * tid + 1] = ex0 + h0;
@@ -1191,19 +1206,18 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   const uint32_t sub = (uint32_t)local % kSureSubs;
   GL uint32_t* sctr = gp(a.sctr) + sub * kCtrStride;
   // A block with several tiles (throughput shapes) loads each next tile's
-  // keys, digits and parent counts while it compacts the current one, so a
+  // key words and parent counts while it compacts the current one, so a
   // tile costs one round trip (its reservation atomic) instead of three; the
   // loads are speculative within the buffers' K*B capacity and masked when
   // used, as the first tile's (issued in the prologue).
-  uint32_t c_key = key_first, c_dg = dg_first, c_pv = pv_first;
+  uint32_t c_key = key_first, c_pv = pv_first;
   for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
     const uint32_t x = base + tid;
     const bool inb = x < N;
     const uint32_t xn = x + (uint32_t)nblk * blockDim.x;
-    uint32_t n_key = kNoScore, n_dg = (uint32_t)kNB, n_pv = 0u;
+    uint32_t n_key = kNoScore, n_pv = 0u;
     if (base + (uint32_t)nblk * blockDim.x < N && xn < cap) {
       n_key = keys[xn];
-      n_dg = (uint32_t)kdig[xn];
       n_pv = (uint32_t)pvalid[xn / B];
     }
     // A slot holds a key only below its parent's valid count.
@@ -1211,11 +1225,9 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     const bool has = inb && (x - px * B) < c_pv;
     const uint32_t sb = has ? c_key : kNoScore;
     const u64 key = sb != kNoScore ? (((u64)sb << 32) | x) : ~0ull;
-    const uint32_t dg = has ? c_dg : (uint32_t)kNB;
     c_key = n_key;
-    c_dg = n_dg;
     c_pv = n_pv;
-    const uint32_t d = key != ~0ull ? dg : (uint32_t)kNB;
+    const uint32_t d = key == ~0ull ? (uint32_t)kNB : (exact ? exact_digit(sb, dlo) : digit1(sb, dlo, ds1));
     const bool sure = d < T1;
     const bool tie = d == T1;
     const u64 bs = __ballot(sure), bt = __ballot(tie);

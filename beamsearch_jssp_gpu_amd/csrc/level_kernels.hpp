@@ -82,13 +82,14 @@ struct LevelArgs {
   const int* beam_cur;       // P x S
   int* beam_next;            // K x S
   int* cbuf;                 // single-phase children, (K*B) x S
-  unsigned long long* keys;  // K*B 8-byte slots: child x's FP32 score bits as u32 at word x (the filter
-                             // forms the (score, x) key), then the refinement scratch (u64)
+  unsigned long long* keys;  // K*B 8-byte slots: child x's key word as u32 at word x (the filter forms the
+                             // (key word, x) key), then the refinement scratch (u64).  Key word: the FP32
+                             // score bits, or with the exact digit (cmax << 11 | the score's ulps above
+                             // fl(cmax / ns)), which orders the children the same way
   unsigned long long* crec;  // K*B: kept child's record, finish | machine << 32 | job << 40 | next
                              // machine << 48 (two buffers by level parity)
   unsigned long long* cand;  // K*B: candidate survivors (first Kp after the filter)
   unsigned long long* ref_buf;  // K*B + K: scratch of the threshold cut / the ties of each filter tile
-  uint16_t* kdig;            // K*B: first selection digit of each key
   uint8_t* pvalid;           // K: valid child slots of each parent (slots >= it hold no key)
   uint32_t* tcnt;            // ceil(K*B/kFilterThreads): keys of that bin per filter tile
   uint32_t* hist;            // kHistCopies x kNB, this level

@@ -20,7 +20,6 @@ struct Work {
   unsigned long long* crec[2];
   unsigned long long* cand;
   unsigned long long* ref_buf;
-  uint16_t* kdig;
   uint8_t* pvalid;
   uint32_t* tcnt;
   uint32_t* hist[2];

@@ -41,7 +41,7 @@ def test_profiled_solve_times_every_launch(engine, phases):
         assert abs(float(b.sum()) - b_tot) <= 1e-6 * b_tot
         if k == RANK:
             # 4S bytes read and written per survivor plus its 8-byte record,
-            # and (fused) the next level's 8-byte child keys.
+            # and (fused) the next level's 12 bytes per child (key word, record).
             base = 8.0 * S * pc[1:L] + 8.0 * pc[1:L]
             assert (b >= base - 1e-6).all()
             if phases == 1:
