@@ -158,10 +158,12 @@ Shape make_shape(const jssp_instance* in, const jssp_params* p, int lanes, int r
   const int parents_per_block = kScoreWaves * (64 / s.G);
   s.nblk_score = (int)std::max(1, std::min((s.K + parents_per_block - 1) / parents_per_block, 512));
   // Throughput shapes (batches, the server): at most 64 filter blocks per
-  // instance, each taking several 1024-child tiles, so fewer blocks re-read
-  // the instance's histogram copies.
+  // instance, each taking several 1024-child tiles, kFilterTilesTp per pass,
+  // so fewer blocks re-read the instance's histogram copies.
   const long long cap = filter_cap > 0 ? filter_cap : (rank_waves == kRankWavesThroughput ? 64 : 512);
-  s.nblk_filter = (int)std::max(1ll, std::min((N + kFilterThreads - 1) / kFilterThreads, cap));
+  const long long tiles = (N + kFilterThreads - 1) / kFilterThreads;
+  const long long per = rank_waves == kRankWavesThroughput ? kFilterTilesTp : 1;
+  s.nblk_filter = (int)std::max(1ll, std::min((tiles + per - 1) / per, cap));
   s.rank_waves = rank_waves == kRankWavesThroughput ? kRankWavesThroughput : kRankWaves;
   s.rank_batch = s.rank_waves == kRankWavesThroughput ? std::max(1, std::min(rank_batch, kRankBatch)) : 1;
   s.nblk_rank = rank_blocks(s.K, s.rank_waves, s.rank_batch);

@@ -886,7 +886,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
 // ---------------------------------------------------------------------------
 // 8-bit MSD radix select over the full 64-bit key, by one block: keeps the
 // `need` smallest of the t keys of R in R[0, need) (order unspecified).
-__device__ void refine_threshold(u64* R, uint32_t t, uint32_t need, u64* scratch) {
+__device__ __forceinline__ void refine_threshold(u64* R, uint32_t t, uint32_t need, u64* scratch) {
   __shared__ u64 s_mn[16], s_mx[16];
   __shared__ uint32_t s_h[256];
   __shared__ u64 s_want, s_mask;
@@ -960,7 +960,7 @@ __device__ void refine_threshold(u64* R, uint32_t t, uint32_t need, u64* scratch
 // index.  Leaves the `need` smallest of the t keys of R in R[0, need).
 constexpr int kSubBins = 4096;
 constexpr int kLog2SubBins = 12;
-__device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bufB, u64* outb,
+__device__ __forceinline__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bufB, u64* outb,
                            uint32_t* bitmap, uint32_t bm_words, uint32_t N) {
   __shared__ uint32_t s_sub[kSubBins];
   __shared__ uint32_t s_mn[16], s_mx[16], s_w[16];
@@ -1101,13 +1101,19 @@ __device__ void refine_cut(u64* R, uint32_t t, uint32_t need, u64* bufA, u64* bu
 // tcnt, and the rank kernel takes the first `need` of them tile by tile.
 // Otherwise the bin's keys are compacted too and the last block to finish
 // cuts them, so that cand[0, Kp) holds the Kp survivors.
+// U: tiles a block pass covers (its threads hold one key of each): the
+// pass's sure keys of all U tiles take one reservation, so a block with
+// several tiles makes 1/U as many dependent round trips.  The single-solve
+// form uses U = 1 (one tile per block at the benchmark sizes), the batch and
+// server form U = kFilterTilesTp.
+template <int U>
 __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   __shared__ uint32_t s_ex[kNB];
   __shared__ uint32_t s_w[16];
   __shared__ uint32_t s_e[16];
   __shared__ int s_km[16];
   __shared__ uint32_t s_T;
-  __shared__ uint32_t s_off[2][16];
+  __shared__ uint32_t s_off[2][U][16];
   __shared__ uint32_t s_base[2];
   __shared__ int s_last;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1131,10 +1137,14 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   const uint32_t dlo = __builtin_amdgcn_readfirstlane(sel->lo), ds1 = __builtin_amdgcn_readfirstlane(sel->s1);
   const uint32_t B = (uint32_t)a.B;
   const uint32_t cap = (uint32_t)a.K * B;
-  const uint32_t xf = (uint32_t)local * blockDim.x + tid;
-  const uint32_t key_first = xf < cap ? keys[xf] : kNoScore;
-  const uint32_t pf = xf / B;
-  const uint32_t pv_first = xf < cap ? (uint32_t)pvalid[pf] : 0u;
+  constexpr uint32_t TS = kFilterThreads;  // slots per tile (blockDim.x)
+  uint32_t c_key[U], c_pv[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t xf = (uint32_t)local * (U * TS) + (uint32_t)u * TS + tid;
+    c_key[u] = xf < cap ? keys[xf] : kNoScore;
+    c_pv[u] = xf < cap ? (uint32_t)pvalid[xf / B] : 0u;
+  }
   uint32_t h0 = 0, h1 = 0;
 #pragma unroll
   for (int c = 0; c < kHistCopies; ++c) {
@@ -1205,57 +1215,97 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   const bool subs = sure_subs(single, a.K, a.B);
   const uint32_t sub = (uint32_t)local % kSureSubs;
   GL uint32_t* sctr = gp(a.sctr) + sub * kCtrStride;
-  // A block with several tiles (throughput shapes) loads each next tile's
+  // A block with several passes (throughput shapes) loads each next pass's
   // key words and parent counts while it compacts the current one, so a
-  // tile costs one round trip (its reservation atomic) instead of three; the
+  // pass costs one round trip (its reservation atomic) instead of three; the
   // loads are speculative within the buffers' K*B capacity and masked when
-  // used, as the first tile's (issued in the prologue).
-  uint32_t c_key = key_first, c_pv = pv_first;
-  for (uint32_t base = (uint32_t)local * blockDim.x; base < N; base += (uint32_t)nblk * blockDim.x) {
-    const uint32_t x = base + tid;
-    const bool inb = x < N;
-    const uint32_t xn = x + (uint32_t)nblk * blockDim.x;
-    uint32_t n_key = kNoScore, n_pv = 0u;
-    if (base + (uint32_t)nblk * blockDim.x < N && xn < cap) {
-      n_key = keys[xn];
-      n_pv = (uint32_t)pvalid[xn / B];
-    }
-    // A slot holds a key only below its parent's valid count.
-    const uint32_t px = x / B;
-    const bool has = inb && (x - px * B) < c_pv;
-    const uint32_t sb = has ? c_key : kNoScore;
-    const u64 key = sb != kNoScore ? (((u64)sb << 32) | x) : ~0ull;
-    c_key = n_key;
-    c_pv = n_pv;
-    const uint32_t d = key == ~0ull ? (uint32_t)kNB : (exact ? exact_digit(sb, dlo) : digit1(sb, dlo, ds1));
-    const bool sure = d < T1;
-    const bool tie = d == T1;
-    const u64 bs = __ballot(sure), bt = __ballot(tie);
-    if (lane == 0) {
-      s_off[0][wave] = (uint32_t)__popcll(bs);
-      s_off[1][wave] = (uint32_t)__popcll(bt);
-    }
-    lds_barrier();
-    if (tid == 0) {
-      uint32_t ts = 0, tt = 0;
-      for (int w = 0; w < 16; ++w) {
-        const uint32_t cs = s_off[0][w], ct = s_off[1][w];
-        s_off[0][w] = ts; s_off[1][w] = tt;
-        ts += cs; tt += ct;
+  // used, as the first pass's (issued in the prologue).
+  const uint32_t stride = (uint32_t)nblk * (U * TS);
+  for (uint32_t base = (uint32_t)local * (U * TS); base < N; base += stride) {
+    bool sure[U], tie[U];
+    uint32_t kw[U], n_key[U], n_pv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t xn = base + stride + (uint32_t)u * TS + tid;
+      n_key[u] = kNoScore;
+      n_pv[u] = 0u;
+      if (base + stride < N && xn < cap) {
+        n_key[u] = keys[xn];
+        n_pv[u] = (uint32_t)pvalid[xn / B];
       }
-      if (subs) s_base[0] = ts ? sub * (uint32_t)a.K + dev_add(sctr, ts) : 0u;
-      else s_base[0] = ts ? dev_add(&sel->sure_ctr, ts) : 0u;
-      if (single) gp(a.tcnt)[base / blockDim.x] = tt;
-      else s_base[1] = tt ? cum_lt + dev_add(&sel->tie_ctr, tt) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t x = base + (uint32_t)u * TS + tid;
+      // A slot holds a key only below its parent's valid count.
+      const uint32_t px = x / B;
+      const bool has = x < N && (x - px * B) < c_pv[u];
+      kw[u] = has ? c_key[u] : kNoScore;
+      c_key[u] = n_key[u];
+      c_pv[u] = n_pv[u];
+      const uint32_t d = kw[u] == kNoScore ? (uint32_t)kNB : (exact ? exact_digit(kw[u], dlo) : digit1(kw[u], dlo, ds1));
+      sure[u] = d < T1;
+      tie[u] = d == T1;
+      const u64 bs = __ballot(sure[u]), bt = __ballot(tie[u]);
+      if (lane == 0) {
+        s_off[0][u][wave] = (uint32_t)__popcll(bs);
+        s_off[1][u][wave] = (uint32_t)__popcll(bt);
+      }
     }
     lds_barrier();
-    if (sure) cand[s_base[0] + s_off[0][wave] + lanes_below(bs, lane)] = key;
-    if (tie) {
-      // Single-valued bin: the tile's ties in child-index order at the
-      // tile's own offset (no atomics); otherwise compacted after the sure keys.
-      const uint32_t o = s_off[1][wave] + lanes_below(bt, lane);
-      if (single) tiles_keys[base + o] = key;
-      else cand[s_base[1] + o] = key;
+    // One reservation for the pass's sure keys (tile by tile, wave by wave);
+    // the ties per tile (single-valued bin: at the tile's own offset, count in
+    // tcnt) or after the sure keys with one reservation.
+    if constexpr (U == 1) {
+      if (tid == 0) {  // thread 0 alone: 16 counts
+        uint32_t ts = 0, tt = 0;
+        for (int w = 0; w < 16; ++w) {
+          const uint32_t cs = s_off[0][0][w], ct = s_off[1][0][w];
+          s_off[0][0][w] = ts;
+          s_off[1][0][w] = tt;
+          ts += cs;
+          tt += ct;
+        }
+        if (subs) s_base[0] = ts ? sub * (uint32_t)a.K + dev_add(sctr, ts) : 0u;
+        else s_base[0] = ts ? dev_add(&sel->sure_ctr, ts) : 0u;
+        if (single) gp(a.tcnt)[base / TS] = tt;
+        else s_base[1] = tt ? cum_lt + dev_add(&sel->tie_ctr, tt) : 0u;
+      }
+    } else if (wave == 0) {
+      // A scan over the (tile, wave) counts, one lane each.
+      static_assert(16 * U <= 64, "one lane per (tile, wave)");
+      const int u = lane >> 4, w = lane & 15;
+      const bool in = u < U;
+      const uint32_t cs = in ? s_off[0][in ? u : 0][w] : 0u, ct = in ? s_off[1][in ? u : 0][w] : 0u;
+      const uint32_t is = wave_incl_scan(cs, lane), it = wave_incl_scan(ct, lane);
+      const uint32_t ts = (uint32_t)__shfl((int)is, 63), tt = (uint32_t)__shfl((int)it, 63);
+      const uint32_t seg = (uint32_t)__shfl((int)it, max(0, 16 * u - 1));
+      const uint32_t seg0 = u > 0 ? seg : 0u;  // ties of the pass's tiles before this one
+      if (in) {
+        s_off[0][u][w] = is - cs;
+        s_off[1][u][w] = single ? it - ct - seg0 : it - ct;
+        if (single && w == 15 && base + (uint32_t)u * TS < N) gp(a.tcnt)[(base + (uint32_t)u * TS) / TS] = it - seg0;
+      }
+      if (lane == 0) {
+        if (subs) s_base[0] = ts ? sub * (uint32_t)a.K + dev_add(sctr, ts) : 0u;
+        else s_base[0] = ts ? dev_add(&sel->sure_ctr, ts) : 0u;
+        if (!single) s_base[1] = tt ? cum_lt + dev_add(&sel->tie_ctr, tt) : 0u;
+      }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t x = base + (uint32_t)u * TS + tid;
+      const u64 key = ((u64)kw[u] << 32) | x;
+      const u64 bs = __ballot(sure[u]), bt = __ballot(tie[u]);
+      if (sure[u]) cand[s_base[0] + s_off[0][u][wave] + lanes_below(bs, lane)] = key;
+      if (tie[u]) {
+        // Single-valued bin: the tile's ties in child-index order at the
+        // tile's own offset (no atomics); otherwise compacted after the sure keys.
+        const uint32_t o = s_off[1][u][wave] + lanes_below(bt, lane);
+        if (single) tiles_keys[base + (uint32_t)u * TS + o] = key;
+        else cand[s_base[1] + o] = key;
+      }
     }
     lds_barrier();
   }
@@ -1996,6 +2046,7 @@ __global__ __launch_bounds__(1024) void k_expand_b(InstDesc* desc, const BlockRe
   score_body<true>(a, r.y, nblk);
 }
 
+template <int U>
 __device__ __forceinline__ void filter_step(InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   const InstDesc d = fetch_desc(desc + r.x);
@@ -2004,17 +2055,17 @@ __device__ __forceinline__ void filter_step(InstDesc* desc, const BlockRef* map)
   else advance(desc[r.x].lvl_score, d.lvl_score, r.y, d.base.L);
   if (level >= d.base.L - 1) return;  // the last level only records the best child
   const LevelArgs a = resolve(d, level);
-  filter_body(a, r.y, d.nblk_filter);
+  filter_body<U>(a, r.y, d.nblk_filter);
 }
 
 __global__ __launch_bounds__(kFilterThreads) void k_filter_b(InstDesc* desc, const BlockRef* map) {
-  filter_step(desc, map);
+  filter_step<1>(desc, map);
 }
 
 // Batches and the server: registers held to 64 per lane (a few spills), so
 // two blocks share a CU and one block's memory waits overlap another's work.
 __global__ __launch_bounds__(kFilterThreads, 8) void k_filter_tp_b(InstDesc* desc, const BlockRef* map) {
-  filter_step(desc, map);
+  filter_step<kFilterTilesTp>(desc, map);
 }
 
 // Rank + materialise; instances of the other form return at once (a batch

@@ -23,6 +23,9 @@ constexpr uint32_t kLevel0Shift = 20;
 constexpr int kScoreWaves = 16;
 // Threads per filter block (one tile of keys per pass) and waves per rank block.
 constexpr int kFilterThreads = 1024;
+// Tiles a block of the batch / server filter form covers per pass (one
+// reservation round trip for all of them).
+constexpr int kFilterTilesTp = 4;
 constexpr int kRankWaves = 16;
 // Rank blocks of batches and the server: half the waves, so two blocks share
 // a CU and one block's memory waits overlap another's work (single solves
