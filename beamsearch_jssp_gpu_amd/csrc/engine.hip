@@ -330,9 +330,9 @@ size_t rank_lds_bytes(const Shape& s) { return rank_lds_for(s.K, s.n, s.m, s.fus
 
 int rank_variant(const Shape& s, const jssp_params* p) {
   if (!s.fused) return s.K > kRankLdsMax ? kRankUnfusedWide : kRankUnfused;
-  const bool small = s.S <= 128 && s.n <= 64;
-  if (p->mode == JSSP_MODE_PMS_GPU) return small ? kRankFusedPmsSmall : kRankFusedPms;
-  return small ? kRankFusedIchSmall : kRankFusedIch;
+  const int sz = rank_size_class(s.S, s.n);
+  if (p->mode == JSSP_MODE_PMS_GPU) return sz == 0 ? kRankFusedPmsSmall : (sz == 1 ? kRankFusedPmsMedium : kRankFusedPms);
+  return sz == 0 ? kRankFusedIchSmall : (sz == 1 ? kRankFusedIchMedium : kRankFusedIch);
 }
 
 int upload_ops(jssp_engine* e, const jssp_instance* in, int2* d_ops) {

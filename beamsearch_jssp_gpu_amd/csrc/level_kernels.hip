@@ -1516,13 +1516,15 @@ __device__ __forceinline__ void count_below_n(const u64* C, int n, const u64 (&k
 // ICH_DET, 2 = PMS_GPU (fused kernels are compiled per mode).  Fused: after
 // materialising a survivor, the wave writes its words to its LDS view and
 // scores its children for the next level.
-// SMALL (fused only): S <= 128 and n <= 64, so a snapshot is two words per
-// lane and a job one slot per lane.  W: waves per block (kRankWaves for
-// single solves, kRankWavesThroughput for batches and the server).
-template <bool FUSED, int MODE, bool SMALL, int W, int NB, bool WIDE = false>
+// SZ (fused only; the snapshot size class, rank_size_class): 0 = S <= 128
+// and n <= 64, so a snapshot is two words per lane and a job one slot per
+// lane; 1 = S <= 256 (four words per lane); 2 = larger.  W: waves per block
+// (kRankWaves for single solves, kRankWavesThroughput for batches and the
+// server).
+template <bool FUSED, int MODE, int SZ, int W, int NB, bool WIDE = false>
 __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nblk) {
-  constexpr int NWM = SMALL ? 2 : kMaxWPL;
-  constexpr int NHM = SMALL ? 1 : 2;
+  constexpr int NWM = SZ == 0 ? 2 : (SZ == 1 ? 4 : kMaxWPL);
+  constexpr int NHM = SZ == 0 ? 1 : 2;
   extern __shared__ u64 s_c[];
   __shared__ uint32_t s_h[kNB];
   __shared__ int s_mc[kRankWaves];
@@ -2081,20 +2083,22 @@ __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? 6 : 4) void k_r
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<false, 0, false, W, NB, WIDE>(a, nullptr, r.y, nblk);
+  rank_body<false, 0, 2, W, NB, WIDE>(a, nullptr, r.y, nblk);
 }
 
-// Rank + materialise + score of the next level (fused form), per mode.
-// Throughput blocks: four per CU for the small variant (64 registers), three
-// for the large one (its spills at 64 cost more than the fourth block gains).
-template <int MODE, bool SMALL, int W, int NB>
-__global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SMALL ? 8 : 6) : 4) void k_rank_fused_b(
+// Rank + materialise + score of the next level (fused form), per mode and
+// snapshot size class.  Throughput blocks: four per CU for the small class
+// (64 registers), three for the medium and large ones (80 registers: their
+// spills at 64 cost more than the fourth block gains; the medium class, four
+// snapshot words per lane instead of eight, then spills nothing).
+template <int MODE, int SZ, int W, int NB>
+__global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SZ == 0 ? 8 : 6) : 4) void k_rank_fused_b(
     InstDesc* desc, const BlockRef* map) {
   const BlockRef r = block_ref(map);
   InstDesc& d = desc[r.x];
   // one round of loads for the variant test (no short-circuit)
-  const bool small = (d.base.S <= 128) & (d.base.n <= 64);
-  if (!((d.fused != 0) & (d.base.mode == MODE) & (small == SMALL))) return;
+  const int sz = rank_size_class(d.base.S, d.base.n);
+  if (!((d.fused != 0) & (d.base.mode == MODE) & (sz == SZ))) return;
   const int level = d.lvl_rank;
   // descriptor fields are read before the counter store (scalar loads)
   const LevelArgs a = resolve(d, level);
@@ -2102,7 +2106,7 @@ __global__ __launch_bounds__(64 * W, W == kRankWavesThroughput ? (SMALL ? 8 : 6)
   const int nblk = d.nblk_rank;
   advance(d.lvl_filter, d.lvl_filter, r.y, a.L);
   if (level >= a.L - 1) return;
-  rank_body<true, MODE, SMALL, W, NB>(a, crec_next, r.y, nblk);
+  rank_body<true, MODE, SZ, W, NB>(a, crec_next, r.y, nblk);
 }
 
 // Folds a served request's per-block partials into entry 0 and clears the
@@ -2253,10 +2257,12 @@ hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, siz
 template <int W, int NB>
 static void (*rank_kernel(int variant))(InstDesc*, const BlockRef*) {
   switch (variant) {
-    case kRankFusedIch: return k_rank_fused_b<1, false, W, NB>;
-    case kRankFusedPms: return k_rank_fused_b<2, false, W, NB>;
-    case kRankFusedIchSmall: return k_rank_fused_b<1, true, W, NB>;
-    case kRankFusedPmsSmall: return k_rank_fused_b<2, true, W, NB>;
+    case kRankFusedIch: return k_rank_fused_b<1, 2, W, NB>;
+    case kRankFusedPms: return k_rank_fused_b<2, 2, W, NB>;
+    case kRankFusedIchSmall: return k_rank_fused_b<1, 0, W, NB>;
+    case kRankFusedPmsSmall: return k_rank_fused_b<2, 0, W, NB>;
+    case kRankFusedIchMedium: return k_rank_fused_b<1, 1, W, NB>;
+    case kRankFusedPmsMedium: return k_rank_fused_b<2, 1, W, NB>;
     case kRankUnfusedWide: return k_rank_b<W, NB, true>;
     default: return k_rank_b<W, NB, false>;
   }

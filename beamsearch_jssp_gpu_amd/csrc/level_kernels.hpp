@@ -172,18 +172,25 @@ hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool
 // throughput: the batch / server form of the kernel (two blocks per CU).
 hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, bool throughput,
                               hipStream_t s, const EventPair* ev = nullptr);
+// Snapshot size class of the fused rank kernels: 0 (small) = S <= 128 words
+// and n <= 64 jobs, 1 (medium) = S <= 256 words, 2 (large) = the rest.
+__host__ __device__ inline int rank_size_class(int S, int n) {
+  return (S <= 128 && n <= 64) ? 0 : (S <= 256 ? 1 : 2);
+}
 // Rank kernel variants: the separate-score form, and the fused form per mode
-// (ICH_DET, PMS_GPU) and size (small: S <= 128 words and n <= 64 jobs).  Each
-// returns at once for instances of another variant, so a batch mixing them
-// launches each variant it holds.
+// (ICH_DET, PMS_GPU) and snapshot size class.  Each returns at once for
+// instances of another variant, so a batch mixing them launches each
+// variant it holds.
 enum RankVariant : int {
   kRankUnfused = 0,
-  kRankFusedIch = 1,
+  kRankFusedIch = 1,  // large
   kRankFusedPms = 2,
   kRankFusedIchSmall = 3,
   kRankFusedPmsSmall = 4,
   kRankUnfusedWide = 5,  // separate-score form with K > kRankLdsMax (survivors ranked from global memory)
-  kRankVariants = 6
+  kRankFusedIchMedium = 6,
+  kRankFusedPmsMedium = 7,
+  kRankVariants = 8
 };
 // waves: kRankWaves or kRankWavesThroughput (the Shape's rank_waves);
 // rank_batch > 1 (throughput waves only): the kernels that rank kRankBatch
