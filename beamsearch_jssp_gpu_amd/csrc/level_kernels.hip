@@ -108,49 +108,71 @@ __device__ __forceinline__ bool stamp_level(int level) {
 #define FILTER_STAMP(k) ((void)0)
 #endif
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+// Wave-wide reductions and scans on DPP row shifts (a few cycles each) and
+// lane reads, instead of LDS-routed lane permutes (an LDS round trip per
+// step).  Every lane of the wave must be active: a row shift reads the
+// neighbouring lanes' registers whatever their state.
+//
+// Lane i of row r (lanes 16 r .. 16 r + 15) takes the value of lane i - K of
+// its row, or `fill` when that lane is outside the row.
+template <int K>
+__device__ __forceinline__ uint32_t dpp_row_shr(uint32_t v, uint32_t fill) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)fill, (int)v, 0x110 + K, 0xF, 0xF, false);
+}
+// Inclusive scan of op within each row of 16 lanes (id: op's identity):
+// after the shifts by 1, 2, 4 and 8, lane i holds op over its row's lanes
+// up to i (Hillis-Steele).
+template <class Op>
+__device__ __forceinline__ uint32_t row_scan(uint32_t v, uint32_t id, Op op) {
+  v = op(v, dpp_row_shr<1>(v, id));
+  v = op(v, dpp_row_shr<2>(v, id));
+  v = op(v, dpp_row_shr<4>(v, id));
+  v = op(v, dpp_row_shr<8>(v, id));
   return v;
 }
+// op over the whole wave from the row scans: the rows' last lanes.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v, uint32_t id, Op op) {
+  v = row_scan(v, id, op);
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  const uint32_t r3 = (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+  return op(op(r0, r1), op(r2, r3));
+}
+struct OpAdd { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; } };
+struct OpMinU { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return min(a, b); } };
+struct OpMaxU { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return max(a, b); } };
+struct OpOr { __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a | b; } };
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) { return wave_reduce(v, 0u, OpAdd()); }
+__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) { return wave_reduce(v, 0xFFFFFFFFu, OpMinU()); }
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) { return wave_reduce(v, 0u, OpMaxU()); }
+__device__ __forceinline__ uint32_t wave_or(uint32_t v) { return wave_reduce(v, 0u, OpOr()); }
+// Signed minimum through the order-preserving map x ^ 0x80000000.
 __device__ __forceinline__ int wave_min_i(int v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
-  return v;
+  return (int)(wave_min_u((uint32_t)v ^ 0x80000000u) ^ 0x80000000u);
 }
-__device__ __forceinline__ uint32_t wave_min_u(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor(v, o));
-  return v;
-}
-__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o));
-  return v;
-}
+// 64-bit minimum / maximum: the extreme high word, then the extreme low word
+// among the lanes holding it.
 __device__ __forceinline__ u64 wave_min64(u64 v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    u64 w = __shfl_xor(v, o);
-    v = w < v ? w : v;
-  }
-  return v;
+  const uint32_t hi = wave_min_u((uint32_t)(v >> 32));
+  const uint32_t lo = wave_min_u((uint32_t)(v >> 32) == hi ? (uint32_t)v : 0xFFFFFFFFu);
+  return ((u64)hi << 32) | lo;
 }
 __device__ __forceinline__ u64 wave_max64(u64 v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    u64 w = __shfl_xor(v, o);
-    v = w > v ? w : v;
-  }
-  return v;
+  const uint32_t hi = wave_max((uint32_t)(v >> 32));
+  const uint32_t lo = wave_max((uint32_t)(v >> 32) == hi ? (uint32_t)v : 0u);
+  return ((u64)hi << 32) | lo;
 }
+// Inclusive prefix sum over the wave's lanes.
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t t = __shfl_up(v, o);
-    if (lane >= o) v += t;
-  }
-  return v;
+  v = row_scan(v, 0u, OpAdd());
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)v, 15);
+  const uint32_t r1 = (uint32_t)__builtin_amdgcn_readlane((int)v, 31);
+  const uint32_t r2 = (uint32_t)__builtin_amdgcn_readlane((int)v, 47);
+  const int row = lane >> 4;
+  return v + (row > 0 ? r0 : 0u) + (row > 1 ? r1 : 0u) + (row > 2 ? r2 : 0u);
 }
 // Block barrier ordering LDS accesses only.  __syncthreads() is also a
 // workgroup-scope release of global memory, so it waits for every load and
@@ -500,11 +522,15 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
     const uint32_t hsh = pivot_hash(a.lseed + (uint32_t)p);
     const int tpos = v.r > 0 ? (int)(hsh % (uint32_t)v.r) : -1;
     const u64 b0 = __ballot(v.rdy[0] && v.pos[0] == tpos) & gmask;
-    const int t0 = __shfl(v.mc[0], b0 ? __ffsll((long long)b0) - 1 : lane);
+    // A whole-wave group reads the pivot lane's machine with a lane read
+    // (the lane is uniform), smaller groups through a permute.
+    const int t0 = G == 64 ? __builtin_amdgcn_readlane(v.mc[0], b0 ? __ffsll((long long)b0) - 1 : 0)
+                           : __shfl(v.mc[0], b0 ? __ffsll((long long)b0) - 1 : lane);
     int tmc = b0 ? t0 : -1;
     if (NHM > 1) {
       const u64 b1 = __ballot(v.rdy[1] && v.pos[1] == tpos) & gmask;
-      const int t1 = __shfl(v.mc[1], b1 ? __ffsll((long long)b1) - 1 : lane);
+      const int t1 = G == 64 ? __builtin_amdgcn_readlane(v.mc[1], b1 ? __ffsll((long long)b1) - 1 : 0)
+                             : __shfl(v.mc[1], b1 ? __ffsll((long long)b1) - 1 : lane);
       tmc = b0 ? t0 : (b1 ? t1 : -1);
     }
 #pragma unroll
@@ -519,9 +545,14 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
         else phi |= 1ull << (v.pos[h] - 64);
       }
     }
-    for (int o = 1; o < G; o <<= 1) {
-      plo |= __shfl_xor(plo, o);
-      if (NHM > 1) phi |= __shfl_xor(phi, o);
+    if (G == 64) {
+      plo = ((u64)wave_or((uint32_t)(plo >> 32)) << 32) | wave_or((uint32_t)plo);
+      if (NHM > 1) phi = ((u64)wave_or((uint32_t)(phi >> 32)) << 32) | wave_or((uint32_t)phi);
+    } else {
+      for (int o = 1; o < G; o <<= 1) {
+        plo |= __shfl_xor(plo, o);
+        if (NHM > 1) phi |= __shfl_xor(phi, o);
+      }
     }
 #pragma unroll
     for (int h = 0; h < NHM; ++h) {
