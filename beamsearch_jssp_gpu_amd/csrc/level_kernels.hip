@@ -222,7 +222,7 @@ __device__ __forceinline__ void lds_count(uint32_t* hist, uint32_t b, bool in, i
   const u64 act = __ballot(in);
   if (!act) return;
   const int l0 = __ffsll((long long)act) - 1;
-  const uint32_t b0 = (uint32_t)__shfl((int)b, l0);
+  const uint32_t b0 = (uint32_t)__builtin_amdgcn_readlane((int)b, l0);  // l0 is uniform
   if (__ballot(in && b == b0) == act) {
     if (lane == l0) atomicAdd(&hist[b0], (uint32_t)__popcll(act));
   } else if (in) {
@@ -237,7 +237,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* ctr, bool take, int la
   const int l0 = __ffsll((long long)m) - 1;
   uint32_t base = 0;
   if (lane == l0) base = atomicAdd(ctr, (uint32_t)__popcll(m));
-  base = (uint32_t)__shfl((int)base, l0);
+  base = (uint32_t)__builtin_amdgcn_readlane((int)base, l0);  // l0 is uniform
   return base + lanes_below(m, lane);
 }
 
@@ -1309,7 +1309,8 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
       const bool in = u < U;
       const uint32_t cs = in ? s_off[0][in ? u : 0][w] : 0u, ct = in ? s_off[1][in ? u : 0][w] : 0u;
       const uint32_t is = wave_incl_scan(cs, lane), it = wave_incl_scan(ct, lane);
-      const uint32_t ts = (uint32_t)__shfl((int)is, 63), tt = (uint32_t)__shfl((int)it, 63);
+      const uint32_t ts = (uint32_t)__builtin_amdgcn_readlane((int)is, 63);
+      const uint32_t tt = (uint32_t)__builtin_amdgcn_readlane((int)it, 63);
       const uint32_t seg = (uint32_t)__shfl((int)it, max(0, 16 * u - 1));
       const uint32_t seg0 = u > 0 ? seg : 0u;  // ties of the pass's tiles before this one
       if (in) {
@@ -1367,11 +1368,12 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
 // (o wave-uniform).
 template <int NWM = kMaxWPL>
 __device__ __forceinline__ int wave_word(const int (&pw)[kMaxWPL], int o) {
+  o = __builtin_amdgcn_readfirstlane(o);
   const int i = o >> 6;
   int v = 0;
 #pragma unroll
   for (int k = 0; k < NWM; ++k) v = (k == i) ? pw[k] : v;
-  return __shfl(v, o & 63);
+  return __builtin_amdgcn_readlane(v, o & 63);
 }
 
 // Loads parent p's words into pw[i] = word lane + 64 i (zeros past S).
