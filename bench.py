@@ -559,6 +559,7 @@ def main():
         "roofline": roofline_obj(dom, phases, traffic),
         "kernels": {KIND_NAMES[k[0]]: {"symbol": KERNEL_SYMBOLS[phases][k[0]], "total_ms": round(k[1], 3),
                                        "launches": k[2], "avg_us": round(k[1] * 1e3 / max(1, k[2]), 3),
+                                       "algorithmic_bytes_per_launch": round(k[3] / max(1, k[2]), 1),
                                        "algorithmic_GBps": round(k[3] / (k[1] / 1e3) / 1e9, 1) if k[1] else 0}
                     for k in kinds if k[2] > 0},
     }
