@@ -121,14 +121,10 @@ int check_params(jssp_engine* e, const jssp_instance* in, const jssp_params* p) 
 // Workspace planning.
 int default_lanes(int n) { return n <= 16 ? 16 : (n <= 32 ? 32 : 64); }
 
-// Dynamic LDS of the rank kernel: survivor keys (K <= kRankLdsMax), the tile
-// prefix of tie counts (padded to 16 bytes) and, fused, one snapshot view per
-// wave.
+// Dynamic LDS of the rank kernel (rank_lds_base, plus the op table where the
+// kernel stages it).
 static size_t rank_lds_for(int K, int n, int m, bool fused, int waves) {
-  const size_t tiles = ((size_t)K * n + kFilterThreads - 1) / kFilterThreads;
-  size_t b = (K <= kRankLdsMax ? (size_t)K * 8 : 0) + 4 * ((tiles + 1 + 3) & ~(size_t)3);
-  if (fused) b += (size_t)waves * (2 * n + m + 1) * 4;
-  return b;
+  return rank_lds_base(K, n, m, fused, waves) + (rank_ops_in_lds(K, n, m, fused, waves) ? (size_t)n * m * 8 : 0);
 }
 
 bool fused_eligible(const jssp_instance* in, const jssp_params* p) {

@@ -457,9 +457,9 @@ __device__ __forceinline__ JobView view_from_words(GL const int2* ops, const Lay
 // The job view of a parent whose words the wave has written to LDS at vw
 // (G = 64); cmax = the parent's cmax.
 // NHM: job slots per lane the caller supports (1 when n <= 64).
-template <int NHM>
-__device__ __forceinline__ JobView view_from_lds(GL const int2* ops, const Layout& Ly, const int* vw, int cmax,
-                                                 int lane) {
+// ops: the op table in memory or in LDS.
+template <int NHM, class P>
+__device__ __forceinline__ JobView view_from_lds(P ops, const Layout& Ly, const int* vw, int cmax, int lane) {
   JobView v;
   const int n = Ly.n, m = Ly.m;
   const int NH = (n + 63) >> 6;
@@ -1586,6 +1586,16 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   GL const uint32_t* tcnt = gp(a.tcnt);
   GL const SelState* sel = gp(a.sel);
   const uint32_t kp_raw = sel->Kp, sf_raw = sel->sorted_from, single_raw = sel->single;
+  // The op table, when staged, is loaded with this first round (and written
+  // to LDS after the keys' loads are out).
+  const bool ops_lds = FUSED && W == kRankWavesThroughput && rank_ops_in_lds(a.K, a.n, a.m, true, W);
+  constexpr int kOpsPer = (kOpsLdsMax + 64 * W - 1) / (64 * W);
+  int2 opv[kOpsPer];
+#pragma unroll
+  for (int k = 0; k < kOpsPer; ++k) {
+    const int i = tid + k * 64 * W;
+    opv[k] = (ops_lds && i < a.n * a.m) ? load_global(a.ops + i) : make_int2(0, 0);
+  }
   const uint32_t N = (uint32_t)gp(a.pcount)[a.level] * (uint32_t)a.B;
   const uint32_t tc0 = t0 < tiles_max ? tcnt[t0] : 0u;
   // The keys below the threshold bin: one run per counter region when the
@@ -1625,8 +1635,17 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
   const bool ties = sf < Kp;
   uint32_t* s_tp = reinterpret_cast<uint32_t*>(s_c + (in_lds ? a.K : 0));
   const int tiles = (int)((N + kFilterThreads - 1) / kFilterThreads);
-  // Fused: per-wave snapshot views after the tile prefix (16-byte aligned).
+  // Fused: per-wave snapshot views after the tile prefix (16-byte aligned),
+  // then, where staged, the op table (8-byte aligned: W * S words, W = 8).
   int* s_view = reinterpret_cast<int*>(s_tp + ((tiles_max + 1 + 3) & ~3));
+  int2* s_ops = reinterpret_cast<int2*>(s_view + W * (2 * a.n + a.m + 1));
+  if (ops_lds) {
+#pragma unroll
+    for (int k = 0; k < kOpsPer; ++k) {
+      const int i = tid + k * 64 * W;
+      if (i < a.n * a.m) s_ops[i] = opv[k];
+    }
+  }
   if (ties) {
     // Exclusive prefix of the per-tile tie counts (tiles past this level's
     // count hold stale counts and are skipped).
@@ -1755,7 +1774,8 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        JobView v = view_from_lds<NHM>(ops, Ly, vw, cm, lane);
+        JobView v = ops_lds ? view_from_lds<NHM>(static_cast<const int2*>(s_ops), Ly, vw, cm, lane)
+                            : view_from_lds<NHM>(ops, Ly, vw, cm, lane);
         int slot[2], nvalid;
         bool keep[2];
         emit_children<NHM>(dn, sln, q, true, 64, lane, ~0ull, lane, v, s_h, acc, slot, keep, nvalid);

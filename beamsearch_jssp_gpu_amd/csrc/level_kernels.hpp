@@ -39,6 +39,33 @@ constexpr int kRankLdsMax = 16384;
 // Dynamic LDS the fused rank kernel may use (its static LDS is ~9 KB of 160 KB);
 // solves that would need more use the separate score kernel.
 constexpr size_t kRankFusedLdsMax = 144 * 1024;
+// Dynamic LDS of the rank kernel before the op table: survivor keys (K <=
+// kRankLdsMax), the tile prefix of tie counts (padded to 16 bytes) and,
+// fused, one snapshot view per wave.
+__host__ __device__ inline size_t rank_lds_base(int K, int n, int m, bool fused, int waves) {
+  const size_t tiles = ((size_t)K * n + kFilterThreads - 1) / kFilterThreads;
+  size_t b = (K <= kRankLdsMax ? (size_t)K * 8 : 0) + 4 * ((tiles + 1 + 3) & ~(size_t)3);
+  if (fused) b += (size_t)waves * (2 * n + m + 1) * 4;
+  return b;
+}
+// Snapshot size class of the fused rank kernels: 0 (small) = S <= 128 words
+// and n <= 64 jobs, 1 (medium) = S <= 256 words, 2 (large) = the rest.
+__host__ __device__ inline int rank_size_class(int S, int n) {
+  return (S <= 128 && n <= 64) ? 0 : (S <= 256 ? 1 : 2);
+}
+// Fused throughput rank blocks (batches, the server) with K <= 1024 also
+// stage the op table (8 bytes per op, up to kOpsLdsMax ops) after the views:
+// their waves then score the next level's children of their survivors from
+// LDS, not from memory (batch of 256 instances +2.4% solves/s).  Up to
+// K = 1024 the table costs no block per CU (at most ~50 KB per block with
+// the ~9 KB of static LDS, against 3 or 4 blocks in 160 KB); wider beams
+// would lose one (a serving mix with K = 4096 requests staging it: 0.149 ->
+// 0.159 s).  Single-solve blocks do not stage it (measured slower: one
+// survivor per wave does not repay the staging).
+constexpr int kOpsLdsMax = 2048;
+__host__ __device__ inline bool rank_ops_in_lds(int K, int n, int m, bool fused, int waves) {
+  return fused && waves == kRankWavesThroughput && K <= 1024 && n * m <= kOpsLdsMax;
+}
 // The filter reserves the slots of the keys below the threshold bin with
 // atomics on kSureSubs counters (filter block b uses counter b % kSureSubs,
 // its keys going to cand[r*K, (r+1)*K)), kCtrStride words apart, instead of
@@ -172,11 +199,6 @@ hipError_t launch_step_score(InstDesc* desc, const BlockRef* map, int nblk, bool
 // throughput: the batch / server form of the kernel (two blocks per CU).
 hipError_t launch_step_filter(InstDesc* desc, const BlockRef* map, int nblk, size_t lds_bytes, bool throughput,
                               hipStream_t s, const EventPair* ev = nullptr);
-// Snapshot size class of the fused rank kernels: 0 (small) = S <= 128 words
-// and n <= 64 jobs, 1 (medium) = S <= 256 words, 2 (large) = the rest.
-__host__ __device__ inline int rank_size_class(int S, int n) {
-  return (S <= 128 && n <= 64) ? 0 : (S <= 256 ? 1 : 2);
-}
 // Rank kernel variants: the separate-score form, and the fused form per mode
 // (ICH_DET, PMS_GPU) and snapshot size class.  Each returns at once for
 // instances of another variant, so a batch mixing them launches each
