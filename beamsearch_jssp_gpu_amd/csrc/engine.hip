@@ -283,6 +283,7 @@ LevelArgs base_args(const Shape& s, const Work& w, const jssp_params* p) {
   a.bt_parent = -1; a.bt_job = -1;
   a.mode = (p->mode == JSSP_MODE_PMS_GPU) ? 2 : 1;
   a.G = s.G;
+  a.hist_copies = hist_copies_for(s.K);
   return a;
 }
 

@@ -39,6 +39,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "level_kernels.hpp"
 #include "snapshot.hpp"
@@ -611,9 +612,10 @@ __device__ __forceinline__ void emit_children(const ScoreDst& a, const ScoreLeve
 }
 
 // Block partials of a score pass: bstat[slot] and the LDS histogram merged
-// into copy `slot % kHistCopies` of hist.  Every thread of the block calls it.
+// into copy `slot % copies` of hist (copies: 1, 2 or 4).  Every thread of the
+// block calls it.
 __device__ __forceinline__ void finish_score_block(ScoreAcc acc, const uint32_t* s_h, BlockStat* bstat,
-                                                   GL uint32_t* hist, int slot, bool last) {
+                                                   GL uint32_t* hist, int slot, bool last, int copies) {
   __shared__ uint32_t s_cnt[16], s_exp[16];
   __shared__ int s_kmin[16];
   __shared__ u64 s_best[16];
@@ -637,7 +639,7 @@ __device__ __forceinline__ void finish_score_block(ScoreAcc acc, const uint32_t*
     store_global(bstat + slot, b);
   }
   if (!last) {
-    GL uint32_t* hc = hist + (size_t)(slot % kHistCopies) * kNB;
+    GL uint32_t* hc = hist + (size_t)(slot & (copies - 1)) * kNB;
     for (int i = threadIdx.x; i < kNB; i += blockDim.x)
       if (s_h[i]) __hip_atomic_fetch_add(&hc[i], s_h[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -911,7 +913,7 @@ __device__ void score_body(const LevelArgs& a, int local, int nblk) {
   // parent of a wave without a parent, the digit base of the last level.
   if (pb0 >= P) asm volatile("" ::"v"(first.pc), "v"(first.w[0]), "v"(first.w[1]), "v"(first.pred[0]), "v"(first.pred[1]));
   asm volatile("" ::"v"(sl.lo));
-  finish_score_block(acc, s_h, a.bstat, gp(a.hist), local, sl.last);
+  finish_score_block(acc, s_h, a.bstat, gp(a.hist), local, sl.last, a.hist_copies);
 }
 
 // ---------------------------------------------------------------------------
@@ -1153,7 +1155,8 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
   // The next level's histogram (read by the filter one level back) starts at
   // zero before its score pass, which follows this kernel.
   GL uint32_t* hist_next = gp(a.hist_next);
-  for (int i = local * (int)blockDim.x + tid; i < kNB * kHistCopies; i += nblk * (int)blockDim.x) hist_next[i] = 0;
+  const int copies = a.hist_copies;
+  for (int i = local * (int)blockDim.x + tid; i < kNB * copies; i += nblk * (int)blockDim.x) hist_next[i] = 0;
 
   // Every load of the prologue goes out at once: the digit parameters, the
   // histogram copies, one score-block partial per thread (nblk_score <=
@@ -1176,13 +1179,22 @@ __device__ void filter_body(const LevelArgs& a, int local, int nblk) {
     c_key[u] = xf < cap ? keys[xf] : kNoScore;
     c_pv[u] = xf < cap ? (uint32_t)pvalid[xf / B] : 0u;
   }
+  // (one branch on the copy count, so each path's loads go out together)
   uint32_t h0 = 0, h1 = 0;
+  GL const u64* hsrc = reinterpret_cast<GL const u64*>(gp(a.hist)) + tid;
+  auto sum_copies = [&](auto nc) {
+    u64 hv[decltype(nc)::value];
 #pragma unroll
-  for (int c = 0; c < kHistCopies; ++c) {
-    const u64 hv = *reinterpret_cast<GL const u64*>(gp(a.hist) + (size_t)c * kNB + 2 * tid);
-    h0 += (uint32_t)hv;
-    h1 += (uint32_t)(hv >> 32);
-  }
+    for (int c = 0; c < decltype(nc)::value; ++c) hv[c] = hsrc[(size_t)c * (kNB / 2)];
+#pragma unroll
+    for (int c = 0; c < decltype(nc)::value; ++c) {
+      h0 += (uint32_t)hv[c];
+      h1 += (uint32_t)(hv[c] >> 32);
+    }
+  };
+  if (copies == kHistCopies) sum_copies(std::integral_constant<int, kHistCopies>());
+  else if (copies == 2) sum_copies(std::integral_constant<int, 2>());
+  else sum_copies(std::integral_constant<int, 1>());
   uint32_t ex = 0;
   int km = INT_MAX;
   if (tid < a.nblk_score) {
@@ -1802,7 +1814,7 @@ __device__ void rank_body(const LevelArgs& a, u64* crec_next, int local, int nbl
     store_global(a.rstat + local, rs);
   }
   RANK_STAMP(5);
-  if (FUSED) finish_score_block(acc, s_h, a.bstat, gp(a.hist_next), local, sln.last);
+  if (FUSED) finish_score_block(acc, s_h, a.bstat, gp(a.hist_next), local, sln.last, a.hist_copies);
   RANK_STAMP(6);
 }
 
@@ -2063,7 +2075,7 @@ __global__ __launch_bounds__(256) void k_init_b(InstDesc* desc, const BlockRef* 
     bool keep[2];
     emit_children(score_dst(a), sl, 0, true, 64, lane, ~0ull, lane, v, s_h, acc, slot, keep, nvalid);
   }
-  finish_score_block(acc, s_h, a.bstat, gp(a.hist), 0, sl.last);
+  finish_score_block(acc, s_h, a.bstat, gp(a.hist), 0, sl.last, a.hist_copies);
   const BlockStat empty = {0u, 0u, INT_MAX, 0u, ~0ull};
   for (int i = 1 + (int)threadIdx.x; i < a.nblk_score; i += blockDim.x) a.bstat[i] = empty;
 }

@@ -15,6 +15,13 @@ constexpr int kNB = 2048;
 // b into copy b % kHistCopies), which the filter sums: same-address atomics
 // serialise, so spreading them cuts the merge time by the copy count.
 constexpr int kHistCopies = 4;
+// Copies a solve uses: one per 64 score/rank blocks at most (a 16-wave rank
+// block takes 16 survivors, so K / 16 blocks), which keeps the same-address
+// merges per copy at single-solve K = 4096's level while narrower beams read
+// fewer copies in every filter block (a batch of 256 instances with K <= 1024:
+// +1.8% solves/s with one copy).  A function of K alone, so every block shape
+// a server may switch a request to merges into and reads the same copies.
+__host__ __device__ inline int hist_copies_for(int K) { return K <= 1024 ? 1 : (K <= 2048 ? 2 : kHistCopies); }
 // The shift makes the previous level's survivor spread cover ~2^10 bins.
 constexpr int kSpreadBins = 10;
 // Shift used at level 0 (no previous level; at most n children anyway).
@@ -154,6 +161,7 @@ struct LevelArgs {
   int G;                     // lanes per parent in the score kernel: 16, 32 or 64
   uint32_t lseed;
   int bt_parent, bt_job;     // explicit backtrack start (exact modes), -1 = use bstat best
+  int hist_copies;           // histogram copies in use: hist_copies_for(K)
 };
 
 // Device-resident state of one solve in flight (a batch is an array of them).
