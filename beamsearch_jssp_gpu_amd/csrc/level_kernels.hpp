@@ -11,15 +11,16 @@ namespace jssp {
 // form: the score bits above a proven lower bound, shifted so the previous
 // level's survivor spread covers ~2^kSpreadBins bins.
 constexpr int kNB = 2048;
-// The score blocks merge their LDS histograms into kHistCopies copies (block
-// b into copy b % kHistCopies), which the filter sums: same-address atomics
-// serialise, so spreading them cuts the merge time by the copy count.
+// The score blocks merge their LDS histograms into up to kHistCopies copies
+// (block b into copy b % hist_copies_for(K)), which the filter sums:
+// same-address atomics serialise, so spreading them cuts the merge time by
+// the copy count.
 constexpr int kHistCopies = 4;
 // Copies a solve uses: one per 64 score/rank blocks at most (a 16-wave rank
 // block takes 16 survivors, so K / 16 blocks), which keeps the same-address
 // merges per copy at single-solve K = 4096's level while narrower beams read
 // fewer copies in every filter block (a batch of 256 instances with K <= 1024:
-// +1.8% solves/s with one copy).  A function of K alone, so every block shape
+// +0.9% solves/s with one copy).  A function of K alone, so every block shape
 // a server may switch a request to merges into and reads the same copies.
 __host__ __device__ inline int hist_copies_for(int K) { return K <= 1024 ? 1 : (K <= 2048 ? 2 : kHistCopies); }
 // The shift makes the previous level's survivor spread cover ~2^10 bins.
