@@ -73,6 +73,17 @@ def test_det_selection_paths(engine, mode, flags):
         _check_det(engine, inst, 64, mode, 1, seed=s + 100, flags=flags)
 
 
+@pytest.mark.parametrize("K", [1024, 2048, 4096])
+def test_det_histogram_copy_counts(engine, K):
+    """Beams of 1024, 2048 and 4096 merge their score histograms into one,
+    two and four copies (the copy count follows K); each gives the oracle's
+    beams in both modes and both phase forms."""
+    inst = splitmix_instance(7, 15, 8, 1, 99)
+    for mode in (bs.MODE_ICH_DET, bs.MODE_PMS_GPU):
+        _check_det(engine, inst, K, mode, 2, seed=77)
+        _check_det(engine, inst, K, mode, 1, seed=78)
+
+
 def test_det_equal_durations(engine):
     """Unit durations: massive score ties, resolved by child index only."""
     inst = splitmix_instance(3, 12, 6, 1, 1)
