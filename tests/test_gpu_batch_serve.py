@@ -49,6 +49,18 @@ def test_batch_equals_single_solves(engine, phases):
         np.testing.assert_array_equal(g.sequence, w["sequence"])
 
 
+def test_batch_wide_beams_equal_single_solves(engine):
+    """Beams of 1024 (one histogram copy; the throughput rank blocks stage
+    the op table in LDS) and 2048 (two copies; op table read from memory)
+    batched equal their single solves."""
+    jobs = _mixed(5, 8, ns=(10, 15, 20), ms=(5, 10), ks=(1024, 2048))
+    print("batch of", len(jobs), "K", sorted({p["beam_width"] for _, p in jobs}), flush=True)
+    singles = [engine.solve(_b(i), **p) for i, p in jobs]
+    got = engine.solve_batch([_b(i) for i, _ in jobs], [p for _, p in jobs])
+    for g, s in zip(got, singles):
+        _same(g, s)
+
+
 def _shape(lanes, n):
     return lanes if (lanes == bs.LANES_FUSED or lanes >= n) else 64
 
