@@ -184,6 +184,24 @@ def test_serving_alone_equals_in_mix(engine):
     srv.close()
 
 
+def test_serving_wide_beams_equal_alone(engine):
+    """Served requests with beams of 1024 and 2048 (one and two histogram
+    copies, whatever rank block shape the server picks as the load changes)
+    equal their single solves."""
+    jobs = _mixed(6, 10, ns=(10, 15, 20), ms=(5, 10), ks=(256, 1024, 2048))
+    alone = [engine.solve(_b(i), **p) for i, p in jobs]
+    srv = bs.Server(engine, hbm_budget_bytes=1 << 30, max_slots=16)
+    tickets = []
+    for k, (inst, p) in enumerate(jobs):
+        tickets.append(srv.submit(_b(inst), p["beam_width"], p["mode"], p["best_known"], p["seed"]))
+        if k % 4 == 3:
+            srv.step(8)
+    srv.drain()
+    for t, a in zip(tickets, alone):
+        _same(srv.poll(t), a)
+    srv.close()
+
+
 def test_serving_deep_queue_equals_alone(engine):
     """Requests queued at once: those admitted behind eight or more waiting
     take the batched rank form, the last ones the per-survivor form; both
